@@ -1,0 +1,468 @@
+"""Python host side of the MI355X kd-tree 3D-Gaussian ray-render path.
+
+Thin ctypes layer over lib/libkdgrt.so (C ABI in include/kdgrt.h): scene
+generation, proxy meshing and SAH tree build on the host; device scenes,
+plain/debug/explicit-ray renders, the persistent batched launch, the
+autotuner and the serving loop on the GPU.  The GPU entry points raise
+KdgrtError when the HIP library or a device is missing; there is no CPU
+fallback in this package.
+
+Reference entry points mirrored (Kd-treeConverter_v0.9/Kd-treeConverter/):
+  Renderer.init()             <- initCuda()                     cudaRenderer.cu:2438
+  Renderer.setup(...)         <- renderGaussianWithCudaSetup()  cudaRenderer.cu:2557
+  Renderer.frame(...)         <- renderGaussianWithCudaFrame()  cudaRenderer.cu:3099
+  Renderer.cleanup()          <- cleanupCudaResources()         cudaRenderer.cu:3486
+  build_tree(...)             <- build_kd_tree_for_composite_object  Kd-treeConverter.cpp:22
+  Tree.write_kdt / read_kdt   <- dump_/read_kd_tree_...          Kd-treeConverter.cpp:297, 444
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libkdgrt.so")
+
+GAUSSIAN_DTYPE = np.dtype([
+    ("pos", "<f4", 3), ("opacity", "<f4"), ("scale", "<f4", 3), ("k_scale", "<f4"), ("rot", "<f4", 4),
+    ("f_dc", "<f4", 3), ("f_rest", "<f4", 45), ("pad", "<f4", 4)])
+assert GAUSSIAN_DTYPE.itemsize == 256
+NODE_DTYPE = np.dtype([("x", "<u4"), ("y", "<u4")])
+
+OUT_RGBT = 0
+OUT_RGB = 1
+N_COUNTERS = 8
+COUNTER_NAMES = ["node_visits", "leaf_visits", "isect_tests", "hits_found", "blend_ops", "max_sort", "n_eval",
+                 "restarts"]
+
+STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "DEVICE", 4: "IO", 5: "FORMAT", 6: "CAPACITY",
+          7: "NOT_READY", 8: "NO_DEVICE", 9: "CANCELLED", 10: "TIMEOUT"}
+
+
+class KdgrtError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Camera(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("u", C.c_float * 3), ("v", C.c_float * 3), ("n", C.c_float * 3),
+                ("fovy_deg", C.c_double), ("aspect", C.c_double), ("near_c", C.c_double)]
+
+
+class CameraInfo(C.Structure):
+    _fields_ = [("eye", C.c_float * 3), ("u", C.c_float * 3), ("v", C.c_float * 3), ("start", C.c_float * 3),
+                ("step_x", C.c_float), ("step_y", C.c_float)]
+
+
+class BuildParams(C.Structure):
+    _fields_ = [("traversal_cost", C.c_float), ("isect_cost", C.c_float), ("empty_bonus", C.c_float),
+                ("max_level", C.c_uint32), ("min_prims", C.c_uint32), ("force_split", C.c_uint32),
+                ("sah_mode", C.c_int32), ("threads", C.c_int32)]
+
+
+class TreeStats(C.Structure):
+    _fields_ = [("n_nodes", C.c_uint32), ("n_leaves", C.c_uint32), ("n_empty_leaves", C.c_uint32),
+                ("max_depth", C.c_uint32), ("max_leaf_size", C.c_uint32), ("n_offsets", C.c_uint64),
+                ("n_triangles", C.c_uint64), ("n_kept_gaussians", C.c_uint64), ("build_seconds", C.c_double),
+                ("mesh_seconds", C.c_double)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("tile_w", C.c_int32), ("tile_h", C.c_int32), ("stack_depth", C.c_int32),
+                ("hit_capacity", C.c_int32), ("waves_per_cu", C.c_int32), ("block_threads", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class View(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("cam", CameraInfo), ("width", C.c_int32), ("height", C.c_int32),
+                ("out_rgbt", C.c_void_p)]
+
+
+class RaySet(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("origins", C.c_void_p), ("dirs", C.c_void_p), ("n_rays", C.c_uint64),
+                ("out_rgbt", C.c_void_p)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch_views", C.c_int32),
+                ("max_inflight_batches", C.c_int32), ("streams", C.c_int32), ("tuning", Tuning),
+                ("batch_window_ms", C.c_double)]
+
+
+class Request(C.Structure):
+    _fields_ = [("scene_id", C.c_int32), ("cam", CameraInfo), ("width", C.c_int32), ("height", C.c_int32),
+                ("arrival_s", C.c_double), ("deadline_s", C.c_double)]
+
+
+class ResultInfo(C.Structure):
+    _fields_ = [("request_id", C.c_uint64), ("status", C.c_int32), ("arrival_s", C.c_double),
+                ("admitted_s", C.c_double), ("launched_s", C.c_double), ("done_s", C.c_double),
+                ("batch_id", C.c_uint32), ("batch_views", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
+                ("cancelled", C.c_uint64), ("batches", C.c_uint64), ("rays", C.c_uint64), ("busy_s", C.c_double),
+                ("wall_s", C.c_double), ("bytes_resident", C.c_uint64), ("bytes_peak", C.c_uint64)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads lib/libkdgrt.so (built by build.py); raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise KdgrtError(3, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.kdgrt_last_error.restype = C.c_char_p
+        L.kdgrt_version.restype = C.c_char_p
+        L.kdgrt_tree_nodes.restype = C.c_void_p
+        L.kdgrt_tree_offsets.restype = C.c_void_p
+        L.kdgrt_scene_bytes_estimate.restype = C.c_uint64
+        L.kdgrt_scene_bytes_estimate.argtypes = [C.c_uint32, C.c_uint64, C.c_uint64]
+        if hasattr(L, "kdgrt_server_now"):
+            L.kdgrt_server_now.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def check(st):
+    if st != 0:
+        raise KdgrtError(st, lib().kdgrt_last_error().decode(errors="replace"))
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+# ------------------------------------------------------------------ host side
+def generate_gaussians(kind, n, seed):
+    """Synthetic activated Gaussians (reference 256-B layout, scales not inverted)."""
+    g = np.zeros(n, dtype=GAUSSIAN_DTYPE)
+    check(lib().kdgrt_generate_gaussians(C.c_int(kind), C.c_uint64(n), C.c_uint64(seed), _p(g)))
+    return g
+
+
+def invert_scales(g):
+    g = g.copy()
+    check(lib().kdgrt_gaussians_invert_scales(_p(g), C.c_uint64(len(g))))
+    return g
+
+
+def default_build_params():
+    p = BuildParams()
+    check(lib().kdgrt_build_params_default(C.byref(p)))
+    return p
+
+
+def proxy_mesh(g):
+    n = C.c_uint64()
+    check(lib().kdgrt_proxy_mesh(_p(g), C.c_uint64(len(g)), None, None, C.c_uint64(0), C.byref(n), None))
+    xyz = np.zeros((n.value, 3, 3), np.float32)
+    gid = np.zeros(n.value, np.uint32)
+    aabb = np.zeros(6, np.float32)
+    check(lib().kdgrt_proxy_mesh(_p(g), C.c_uint64(len(g)), _p(xyz), _p(gid), C.c_uint64(n.value), C.byref(n),
+                                 _p(aabb)))
+    return xyz, gid, aabb
+
+
+class Tree:
+    """Host kd-tree (nodes, leaf Gaussian-ID lists, scene AABB in .kdt order)."""
+
+    def __init__(self, handle):
+        self.h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+
+    @staticmethod
+    def build(g, params=None, threads=0):
+        p = params or default_build_params()
+        p.threads = threads
+        h = C.c_void_p()
+        check(lib().kdgrt_tree_build(_p(g), C.c_uint64(len(g)), C.byref(p), C.byref(h)))
+        return Tree(h)
+
+    @staticmethod
+    def from_arrays(nodes, offsets, aabb):
+        nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint32)
+        aabb = np.ascontiguousarray(aabb, dtype=np.float32)
+        h = C.c_void_p()
+        check(lib().kdgrt_tree_from_arrays(_p(nodes), C.c_uint32(len(nodes)), _p(offsets) if len(offsets) else None,
+                                           C.c_uint64(len(offsets)), _p(aabb), C.byref(h)))
+        return Tree(h)
+
+    @staticmethod
+    def read_kdt(path):
+        h = C.c_void_p()
+        check(lib().kdgrt_kdt_read(path.encode(), C.byref(h)))
+        return Tree(h)
+
+    def write_kdt(self, path):
+        check(lib().kdgrt_kdt_write(self.h, path.encode()))
+
+    def stats(self):
+        s = TreeStats()
+        check(lib().kdgrt_tree_stats_get(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in TreeStats._fields_}
+
+    @property
+    def nodes(self):
+        n = self.stats()["n_nodes"]
+        ptr = lib().kdgrt_tree_nodes(self.h)
+        return np.ctypeslib.as_array((C.c_uint32 * (2 * n)).from_address(ptr)).view(NODE_DTYPE).copy()
+
+    @property
+    def offsets(self):
+        n = self.stats()["n_offsets"]
+        if n == 0:
+            return np.zeros(0, np.uint32)
+        ptr = lib().kdgrt_tree_offsets(self.h)
+        return np.ctypeslib.as_array((C.c_uint32 * n).from_address(ptr)).copy()
+
+    @property
+    def aabb(self):
+        a = np.zeros(6, np.float32)
+        check(lib().kdgrt_tree_aabb(self.h, _p(a)))
+        return a
+
+    def validate(self, n_gaussians=0):
+        check(lib().kdgrt_tree_validate(self.h, C.c_uint64(n_gaussians)))
+
+    def free(self):
+        if self.h:
+            lib().kdgrt_tree_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def camera_make(cam, width, height):
+    ci = CameraInfo()
+    check(lib().kdgrt_camera_make(C.byref(cam), C.c_int(width), C.c_int(height), C.byref(ci)))
+    return ci
+
+
+def look_at(eye, target, fovy_deg=60.0, aspect=1.0, near=0.005, up=(0.0, 1.0, 0.0)):
+    """Pinhole camera at `eye` looking at `target` (forward = -n)."""
+    eye = np.asarray(eye, np.float64)
+    n = eye - np.asarray(target, np.float64)
+    n /= np.linalg.norm(n)
+    up = np.asarray(up, np.float64)
+    if abs(np.dot(up, n)) > 0.999:
+        up = np.array([0.0, 0.0, 1.0])
+    u = np.cross(up, n)
+    u /= np.linalg.norm(u)
+    v = np.cross(n, u)
+    c = Camera()
+    for k in range(3):
+        c.pos[k], c.u[k], c.v[k], c.n[k] = eye[k], u[k], v[k], n[k]
+    c.fovy_deg, c.aspect, c.near_c = fovy_deg, aspect, near
+    return c
+
+
+def cam_info_bytes(ci):
+    return bytes(ci)
+
+
+# ------------------------------------------------------------------ device side
+class Device:
+    """A kdgrt context on one HIP device plus helpers for device buffers."""
+
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        check(lib().kdgrt_init(C.c_int(device), C.byref(self.h)))
+        self.device = device
+
+    def info(self):
+        name = C.create_string_buffer(256)
+        ncu, mem, lds = C.c_int(), C.c_size_t(), C.c_size_t()
+        check(lib().kdgrt_device_info(self.h, name, C.c_size_t(256), C.byref(ncu), C.byref(mem), C.byref(lds)))
+        return {"name": name.value.decode(), "n_cu": ncu.value, "total_mem": mem.value, "lds_per_block": lds.value}
+
+    def alloc(self, nbytes):
+        p = C.c_void_p()
+        check(lib().kdgrt_device_alloc(self.h, C.c_size_t(nbytes), C.byref(p)))
+        return DeviceBuffer(self, p, nbytes)
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        b = self.alloc(arr.nbytes)
+        check(lib().kdgrt_memcpy_h2d(self.h, b.ptr, _p(arr), C.c_size_t(arr.nbytes)))
+        return b
+
+    def stream(self):
+        s = C.c_void_p()
+        check(lib().kdgrt_stream_create(self.h, C.byref(s)))
+        return s
+
+    def sync(self, stream=None):
+        if stream is None:
+            check(lib().kdgrt_device_sync(self.h))
+        else:
+            check(lib().kdgrt_stream_sync(self.h, stream))
+
+    def timer_begin(self, stream=None):
+        check(lib().kdgrt_timer_begin(self.h, stream))
+
+    def timer_end(self, stream=None):
+        ms = C.c_float()
+        check(lib().kdgrt_timer_end(self.h, stream, C.byref(ms)))
+        return ms.value
+
+    def scene(self, tree, g, scales_inverted=False):
+        return Scene(self, tree, g, scales_inverted)
+
+    def render(self, scene, ci, w, h, out, depth=12, fmt=OUT_RGBT, stream=None):
+        check(lib().kdgrt_render(self.h, scene.h, C.byref(ci), C.c_int(w), C.c_int(h), C.c_int(depth), C.c_int(fmt),
+                                 out.ptr, stream))
+
+    def render_image(self, scene, ci, w, h, depth=12, fmt=OUT_RGBT):
+        ch = 4 if fmt == OUT_RGBT else 3
+        out = self.alloc(w * h * ch * 4)
+        self.render(scene, ci, w, h, out, depth, fmt)
+        self.sync()
+        img = out.download(np.float32, (h, w, ch))
+        out.free()
+        return img
+
+    def render_debug(self, scene, ci, w, h, depth=12, seq_cap=0):
+        out = self.alloc(w * h * 16)
+        cnt = self.alloc(w * h * N_COUNTERS * 4)
+        hsh = self.alloc(w * h * 8)
+        seq = None
+        if seq_cap:
+            seq = self.alloc(w * h * seq_cap * 4)
+            check(lib().kdgrt_memset(self.h, seq.ptr, C.c_int(0xFF), C.c_size_t(w * h * seq_cap * 4)))
+        check(lib().kdgrt_render_debug(self.h, scene.h, C.byref(ci), C.c_int(w), C.c_int(h), C.c_int(depth), out.ptr,
+                                       cnt.ptr, hsh.ptr, seq.ptr if seq else None, C.c_int(seq_cap), None))
+        self.sync()
+        res = {"rgbt": out.download(np.float32, (h, w, 4)),
+               "counters": cnt.download(np.int32, (h, w, N_COUNTERS)),
+               "hash": hsh.download(np.uint64, (h, w))}
+        if seq:
+            res["seq"] = seq.download(np.uint32, (h, w, seq_cap))
+            seq.free()
+        for b in (out, cnt, hsh):
+            b.free()
+        return res
+
+    def render_rays(self, scene, origins, dirs, depth=12, debug=False):
+        origins = np.ascontiguousarray(origins, np.float32)
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        n = len(origins)
+        o, d = self.upload(origins), self.upload(dirs)
+        out = self.alloc(n * 16)
+        res = {}
+        if debug:
+            cnt, hsh = self.alloc(n * 32), self.alloc(n * 8)
+            check(lib().kdgrt_render_rays_debug(self.h, scene.h, o.ptr, d.ptr, C.c_uint64(n), C.c_int(depth), out.ptr,
+                                                cnt.ptr, hsh.ptr, None))
+            self.sync()
+            res["counters"] = cnt.download(np.int32, (n, N_COUNTERS))
+            res["hash"] = hsh.download(np.uint64, (n,))
+            cnt.free()
+            hsh.free()
+        else:
+            check(lib().kdgrt_render_rays(self.h, scene.h, o.ptr, d.ptr, C.c_uint64(n), C.c_int(depth), out.ptr, None))
+            self.sync()
+        res["rgbt"] = out.download(np.float32, (n, 4))
+        for b in (o, d, out):
+            b.free()
+        return res
+
+    def close(self):
+        if self.h:
+            lib().kdgrt_shutdown(self.h)
+            self.h = None
+
+
+class DeviceBuffer:
+    def __init__(self, dev, ptr, nbytes):
+        self.dev, self.ptr, self.nbytes = dev, ptr, nbytes
+
+    def download(self, dtype, shape):
+        a = np.empty(shape, dtype)
+        assert a.nbytes <= self.nbytes
+        check(lib().kdgrt_memcpy_d2h(self.dev.h, _p(a), self.ptr, C.c_size_t(a.nbytes)))
+        return a
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        check(lib().kdgrt_memcpy_h2d(self.dev.h, self.ptr, _p(arr), C.c_size_t(arr.nbytes)))
+
+    def free(self):
+        if self.ptr:
+            lib().kdgrt_device_free(self.dev.h, self.ptr)
+            self.ptr = None
+
+
+class Scene:
+    """Device-resident scene (repacked tree + Gaussians)."""
+
+    def __init__(self, dev, tree, g, scales_inverted=False):
+        self.dev = dev
+        self.h = C.c_void_p()
+        g = np.ascontiguousarray(g, dtype=GAUSSIAN_DTYPE)
+        check(lib().kdgrt_scene_create_from_tree(dev.h, tree.h, _p(g), C.c_uint64(len(g)),
+                                                 C.c_int(1 if scales_inverted else 0), C.byref(self.h)))
+
+    def device_bytes(self):
+        b = C.c_uint64()
+        check(lib().kdgrt_scene_device_bytes(self.h, C.byref(b)))
+        return b.value
+
+    def free(self):
+        if self.h:
+            lib().kdgrt_scene_destroy(self.h)
+            self.h = None
+
+
+def default_tuning(dev=None):
+    t = Tuning()
+    check(lib().kdgrt_tuning_default(dev.h if dev else None, C.byref(t)))
+    return t
+
+
+class Batch:
+    """A persistent batched launch over many views / ray sets."""
+
+    def __init__(self, dev, views, ray_sets=(), tuning=None):
+        self.dev = dev
+        nv, ns = len(views), len(ray_sets)
+        va = (View * max(1, nv))(*views)
+        ra = (RaySet * max(1, ns))(*ray_sets)
+        self.h = C.c_void_p()
+        check(lib().kdgrt_batch_create(dev.h, va if nv else None, C.c_int(nv), ra if ns else None, C.c_int(ns),
+                                       C.byref(tuning) if tuning is not None else None, C.byref(self.h)))
+
+    def run(self, stream=None, use_graph=False):
+        check(lib().kdgrt_batch_run(self.h, stream, C.c_int(1 if use_graph else 0)))
+
+    def info(self):
+        r, t, g = C.c_uint64(), C.c_uint32(), C.c_uint32()
+        check(lib().kdgrt_batch_info(self.h, C.byref(r), C.byref(t), C.byref(g)))
+        return {"n_rays": r.value, "n_tiles": t.value, "grid_blocks": g.value}
+
+    def free(self):
+        if self.h:
+            lib().kdgrt_batch_destroy(self.h)
+            self.h = None
+
+
+def autotune(dev, views, ray_sets=(), max_trials=24, reps=3):
+    nv, ns = len(views), len(ray_sets)
+    va = (View * max(1, nv))(*views)
+    ra = (RaySet * max(1, ns))(*ray_sets)
+    best, ms = Tuning(), C.c_float()
+    check(lib().kdgrt_autotune(dev.h, va if nv else None, C.c_int(nv), ra if ns else None, C.c_int(ns),
+                               C.c_int(max_trials), C.c_int(reps), C.byref(best), C.byref(ms)))
+    return best, ms.value

@@ -1,0 +1,812 @@
+// HIP kernels and device runtime behind the C ABI (include/kdgrt.h).
+//
+// Kernels:
+//  k_view      plain one-view launch (cudaRenderer.cu:2313-2390 + 3099-3484):
+//              256-thread blocks = four 8x8-pixel wave tiles; block index
+//              remapped so each XCD walks a contiguous band of the image
+//              (one L2 per XCD keeps that band's nodes and Gaussians).
+//  k_rays      explicit ray lists, 64 rays per wave.
+//  k_batch     persistent launch over a device work queue of ray tiles drawn
+//              from many views / ray sets / scenes; one queue per XCD with
+//              stealing, each wave pulls its own tile (no block barriers).
+//  k_reset     zeroes the queue counters (graph-safe, no memset node).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+#include "trace_device.hpp"
+
+using namespace kdgrt_dev;
+
+namespace {
+
+constexpr int kXcds = 8;
+
+// ---------------------------------------------------------------------------
+// Plain one-view kernel.
+// ---------------------------------------------------------------------------
+struct ViewArgs {
+    DevScene sc;
+    CamInfo cam;
+    int W, H;
+    int depth;
+    int blocks_x, blocks_y;
+    void* out;
+    int32_t* counters;
+    uint64_t* hashes;
+    uint32_t* seq;
+    int seq_cap;
+};
+
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
+    // Hardware dispatches workgroup b to XCD b % 8.  Give XCD x the
+    // contiguous logical range [x*q + min(x,r), ...) so neighbouring tiles
+    // share an L2.
+    const uint32_t q = nb / kXcds, r = nb % kXcds;
+    const uint32_t x = b % kXcds, i = b / kXcds;
+    return x * q + (x < r ? x : r) + i;
+}
+
+template <int DMAX, int K, bool DBG, int FMT>
+__global__ __launch_bounds__(256) void k_view(ViewArgs a) {
+    const uint32_t nb = (uint32_t)a.blocks_x * (uint32_t)a.blocks_y;
+    const uint32_t lb = xcd_remap(blockIdx.x, nb);
+    const int bx = (int)(lb % (uint32_t)a.blocks_x), by = (int)(lb / (uint32_t)a.blocks_x);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int x = bx * 16 + (wave & 1) * 8 + (lane & 7);
+    const int y = by * 16 + (wave >> 1) * 8 + (lane >> 3);
+    if (x >= a.W || y >= a.H) return;
+    float o[3], d[3], rgb[3], A;
+    make_ray(a.cam, x, y, o, d);
+    const size_t rid = (size_t)y * a.W + x;
+    RayDbg dbg;
+    if (DBG) {
+        for (int i = 0; i < 8; ++i) dbg.cnt[i] = 0;
+        dbg.hash = 0xcbf29ce484222325ull;
+        dbg.seq = a.seq ? a.seq + rid * (size_t)a.seq_cap : nullptr;
+        dbg.seq_cap = a.seq_cap;
+        dbg.nseq = 0;
+    }
+    trace_ray<DMAX, K, DBG>(a.sc, o, d, a.depth, rgb, A, &dbg);
+    const size_t pid = (size_t)(a.H - y - 1) * a.W + x; // framebuffer row flip (cudaRenderer.cu:2367)
+    if (FMT == KDGRT_OUT_RGBT_F32) {
+        reinterpret_cast<float4*>(a.out)[pid] = make_float4(rgb[0], rgb[1], rgb[2], 1.0f - A);
+    } else {
+        float* f = reinterpret_cast<float*>(a.out) + 3 * pid;
+        f[0] = rgb[0];
+        f[1] = rgb[1];
+        f[2] = rgb[2];
+    }
+    if (DBG) {
+        if (a.counters)
+            for (int i = 0; i < 8; ++i) a.counters[8 * rid + i] = dbg.cnt[i];
+        if (a.hashes) a.hashes[rid] = dbg.hash;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Explicit rays.
+// ---------------------------------------------------------------------------
+struct RaysArgs {
+    DevScene sc;
+    const float* ro;
+    const float* rd;
+    uint64_t n;
+    int depth;
+    float4* out;
+    int32_t* counters;
+    uint64_t* hashes;
+};
+
+template <int DMAX, int K, bool DBG>
+__global__ __launch_bounds__(256) void k_rays(RaysArgs a) {
+    const uint32_t nb = gridDim.x;
+    const uint64_t i = (uint64_t)xcd_remap(blockIdx.x, nb) * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    const float o[3] = {a.ro[3 * i], a.ro[3 * i + 1], a.ro[3 * i + 2]};
+    const float d[3] = {a.rd[3 * i], a.rd[3 * i + 1], a.rd[3 * i + 2]};
+    float rgb[3], A;
+    RayDbg dbg;
+    if (DBG) {
+        for (int k = 0; k < 8; ++k) dbg.cnt[k] = 0;
+        dbg.hash = 0xcbf29ce484222325ull;
+        dbg.seq = nullptr;
+        dbg.seq_cap = 0;
+        dbg.nseq = 0;
+    }
+    trace_ray<DMAX, K, DBG>(a.sc, o, d, a.depth, rgb, A, &dbg);
+    a.out[i] = make_float4(rgb[0], rgb[1], rgb[2], 1.0f - A);
+    if (DBG) {
+        if (a.counters)
+            for (int k = 0; k < 8; ++k) a.counters[8 * i + k] = dbg.cnt[k];
+        if (a.hashes) a.hashes[i] = dbg.hash;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent batched kernel.
+// ---------------------------------------------------------------------------
+struct WorkItem { // one view or one explicit ray set
+    DevScene sc;
+    CamInfo cam;
+    int32_t W, H;       // view size (ray sets: W = 0)
+    int32_t tiles_x;    // view tiles per row
+    uint32_t tile_begin;
+    uint32_t tile_count;
+    float4* out;
+    const float* ro;
+    const float* rd;
+    uint64_t n_rays;
+};
+
+struct BatchArgs {
+    const WorkItem* items;
+    int n_items;
+    uint32_t q_begin[kXcds + 1]; // queue x owns tiles [q_begin[x], q_begin[x+1])
+    uint32_t* counters;          // kXcds counters
+    int depth;
+    int tile_w, tile_h;          // view tile in pixels; tile_w*tile_h is a multiple of 64
+};
+
+__global__ void k_reset(uint32_t* c, int n) {
+    if ((int)threadIdx.x < n) c[threadIdx.x] = 0;
+}
+
+template <int DMAX, int K>
+__global__ __launch_bounds__(256) void k_batch(BatchArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int home = blockIdx.x % kXcds;
+    int probe = 0;
+    while (probe < kXcds) {
+        const int q = (home + probe) % kXcds;
+        const uint32_t qlen = a.q_begin[q + 1] - a.q_begin[q];
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(&a.counters[q], 1u);
+        t = __shfl(t, 0);
+        if (t >= qlen) {
+            ++probe; // this queue is drained; move on for good
+            continue;
+        }
+        const uint32_t tile = a.q_begin[q] + t;
+        // locate the work item (items are sorted by tile_begin)
+        int lo = 0, hi = a.n_items - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.items[mid].tile_begin <= tile) lo = mid;
+            else hi = mid - 1;
+        }
+        const WorkItem& it = a.items[lo];
+        const uint32_t local = tile - it.tile_begin;
+        if (it.W > 0) {
+            const int tx = (int)(local % (uint32_t)it.tiles_x), ty = (int)(local / (uint32_t)it.tiles_x);
+            const int npx = a.tile_w * a.tile_h;
+            for (int p = lane; p < npx; p += 64) {
+                // 8x8 sub-squares inside the tile keep each 64-lane step compact
+                const int sub = p >> 6, l = p & 63;
+                const int subs_x = a.tile_w >> 3;
+                const int x = tx * a.tile_w + (sub % subs_x) * 8 + (l & 7);
+                const int y = ty * a.tile_h + (sub / subs_x) * 8 + (l >> 3);
+                if (x >= it.W || y >= it.H) continue;
+                float o[3], d[3], rgb[3], A;
+                make_ray(it.cam, x, y, o, d);
+                trace_ray<DMAX, K, false>(it.sc, o, d, a.depth, rgb, A, nullptr);
+                it.out[(size_t)(it.H - y - 1) * it.W + x] = make_float4(rgb[0], rgb[1], rgb[2], 1.0f - A);
+            }
+        } else {
+            const int npx = a.tile_w * a.tile_h;
+            for (int p = lane; p < npx; p += 64) {
+                const uint64_t i = (uint64_t)local * npx + p;
+                if (i >= it.n_rays) continue;
+                const float o[3] = {it.ro[3 * i], it.ro[3 * i + 1], it.ro[3 * i + 2]};
+                const float d[3] = {it.rd[3 * i], it.rd[3 * i + 1], it.rd[3 * i + 2]};
+                float rgb[3], A;
+                trace_ray<DMAX, K, false>(it.sc, o, d, a.depth, rgb, A, nullptr);
+                it.out[i] = make_float4(rgb[0], rgb[1], rgb[2], 1.0f - A);
+            }
+        }
+    }
+}
+
+typedef void (*BatchKernel)(BatchArgs);
+
+BatchKernel pick_batch_kernel(int dmax, int k) {
+    if (dmax <= 12) {
+        if (k == 4) return k_batch<12, 4>;
+        if (k == 16) return k_batch<12, 16>;
+        return k_batch<12, 8>;
+    }
+    if (k == 4) return k_batch<32, 4>;
+    if (k == 16) return k_batch<32, 16>;
+    return k_batch<32, 8>;
+}
+
+} // namespace
+
+// ===========================================================================
+// Runtime objects.
+// ===========================================================================
+struct kdgrt_context {
+    int device = 0;
+    hipDeviceProp_t prop{};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::mutex mu;
+};
+
+struct kdgrt_scene {
+    kdgrt_context* ctx = nullptr;
+    DevScene dev{};
+    void* d_nodes = nullptr;
+    void* d_offs = nullptr;
+    void* d_geom = nullptr;
+    void* d_shade = nullptr;
+    uint64_t bytes = 0;
+    uint32_t n_nodes = 0;
+    uint64_t n_offsets = 0, n_gaussians = 0;
+};
+
+struct kdgrt_batch {
+    kdgrt_context* ctx = nullptr;
+    kdgrt_tuning tuning{};
+    WorkItem* d_items = nullptr;
+    uint32_t* d_counters = nullptr;
+    BatchArgs args{};
+    BatchKernel kernel = nullptr;
+    int grid = 0, block = 0;
+    uint32_t n_tiles = 0;
+    uint64_t n_rays = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    void* graph_stream = nullptr;
+};
+
+using kdgrt::set_error;
+
+#define HIP_TRY(expr)                                                                                           \
+    do {                                                                                                        \
+        hipError_t e_ = (expr);                                                                                 \
+        if (e_ != hipSuccess)                                                                                   \
+            return set_error(e_ == hipErrorOutOfMemory ? KDGRT_ERR_OUT_OF_MEMORY : KDGRT_ERR_DEVICE,            \
+                             std::string(#expr) + ": " + hipGetErrorString(e_));                                 \
+    } while (0)
+
+namespace kdgrt {
+
+// Launch helpers shared with the serving loop.
+kdgrt_status batch_build(kdgrt_context* ctx, const kdgrt_view* views, int n_views, const kdgrt_ray_set* sets,
+                         int n_sets, const kdgrt_tuning* tuning, kdgrt_batch** out);
+kdgrt_status batch_launch(kdgrt_batch* b, hipStream_t s);
+
+} // namespace kdgrt
+
+extern "C" {
+
+kdgrt_status kdgrt_device_count(int* count) {
+    if (!count) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "device_count: null");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return KDGRT_OK;
+}
+
+kdgrt_status kdgrt_init(int device, kdgrt_context** out) {
+    if (!out) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "init: null");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return set_error(KDGRT_ERR_NO_DEVICE, "init: no HIP device");
+    if (device < 0 || device >= n) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "init: bad device index");
+    HIP_TRY(hipSetDevice(device));
+    auto* c = new kdgrt_context();
+    c->device = device;
+    if (hipGetDeviceProperties(&c->prop, device) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
+        hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return set_error(KDGRT_ERR_DEVICE, "init: device query failed");
+    }
+    *out = c;
+    return KDGRT_OK;
+}
+
+kdgrt_status kdgrt_shutdown(kdgrt_context* ctx) {
+    if (!ctx) return KDGRT_OK;
+    hipSetDevice(ctx->device);
+    hipDeviceSynchronize();
+    if (ctx->ev0) hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) hipEventDestroy(ctx->ev1);
+    delete ctx;
+    return KDGRT_OK;
+}
+
+kdgrt_status kdgrt_device_info(kdgrt_context* ctx, char* name, size_t name_len, int* n_cu, size_t* total_mem,
+                               size_t* lds_per_block) {
+    if (!ctx) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "device_info: null");
+    if (name && name_len) {
+        std::snprintf(name, name_len, "%s (%s)", ctx->prop.name, ctx->prop.gcnArchName);
+    }
+    if (n_cu) *n_cu = ctx->prop.multiProcessorCount;
+    if (total_mem) *total_mem = ctx->prop.totalGlobalMem;
+    if (lds_per_block) *lds_per_block = ctx->prop.sharedMemPerBlock;
+    return KDGRT_OK;
+}
+
+kdgrt_status kdgrt_device_alloc(kdgrt_context* ctx, size_t bytes, void** dptr) {
+    if (!ctx || !dptr) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "device_alloc: null");
+    HIP_TRY(hipSetDevice(ctx->device));
+    *dptr = nullptr;
+    if (bytes == 0) return KDGRT_OK;
+    HIP_TRY(hipMalloc(dptr, bytes));
+    return KDGRT_OK;
+}
+kdgrt_status kdgrt_device_free(kdgrt_context* ctx, void* dptr) {
+    if (!ctx) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "device_free: null");
+    if (!dptr) return KDGRT_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipFree(dptr));
+    return KDGRT_OK;
+}
+kdgrt_status kdgrt_memcpy_h2d(kdgrt_context* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "memcpy_h2d: null");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (bytes) HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return KDGRT_OK;
+}
+kdgrt_status kdgrt_memcpy_d2h(kdgrt_context* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "memcpy_d2h: null");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (bytes) HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return KDGRT_OK;
+}
+kdgrt_status kdgrt_memset(kdgrt_context* ctx, void* dptr, int value, size_t bytes) {
+    if (!ctx || (bytes && !dptr)) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "memset: null");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (bytes) HIP_TRY(hipMemset(dptr, value, bytes));
+    return KDGRT_OK;
+}
+kdgrt_status kdgrt_stream_create(kdgrt_context* ctx, void** stream) {
+    if (!ctx || !stream) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "stream_create: null");
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = (void*)s;
+    return KDGRT_OK;
+}
+kdgrt_status kdgrt_stream_destroy(kdgrt_context* ctx, void* stream) {
+    if (!ctx) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "stream_destroy: null");
+    if (!stream) return KDGRT_OK;
+    HIP_TRY(hipStreamDestroy((hipStream_t)stream));
+    return KDGRT_OK;
+}
+kdgrt_status kdgrt_stream_sync(kdgrt_context* ctx, void* stream) {
+    if (!ctx) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "stream_sync: null");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return KDGRT_OK;
+}
+kdgrt_status kdgrt_device_sync(kdgrt_context* ctx) {
+    if (!ctx) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "device_sync: null");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipDeviceSynchronize());
+    return KDGRT_OK;
+}
+kdgrt_status kdgrt_timer_begin(kdgrt_context* ctx, void* stream) {
+    if (!ctx) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "timer_begin: null");
+    HIP_TRY(hipEventRecord(ctx->ev0, (hipStream_t)stream));
+    return KDGRT_OK;
+}
+kdgrt_status kdgrt_timer_end(kdgrt_context* ctx, void* stream, float* ms) {
+    if (!ctx || !ms) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "timer_end: null");
+    HIP_TRY(hipEventRecord(ctx->ev1, (hipStream_t)stream));
+    HIP_TRY(hipEventSynchronize(ctx->ev1));
+    HIP_TRY(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    return KDGRT_OK;
+}
+
+uint64_t kdgrt_scene_bytes_estimate(uint32_t n_nodes, uint64_t n_offsets, uint64_t n_gaussians) {
+    auto r = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
+    return r(8ull * n_nodes) + r(4ull * n_offsets) + r(48ull * n_gaussians) + r(192ull * n_gaussians);
+}
+
+kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uint32_t n_nodes,
+                                const uint32_t* offsets, uint64_t n_offsets, const float aabb[6],
+                                const kdgrt_gaussian* g, uint64_t n_g, int scales_inverted, kdgrt_scene** out) {
+    if (!ctx || !out || !aabb || (!g && n_g) || (!offsets && n_offsets))
+        return set_error(KDGRT_ERR_INVALID_ARGUMENT, "scene_create: null argument");
+    if (n_g == 0) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "scene_create: no Gaussians");
+    kdgrt_status st = kdgrt::validate_tree(nodes, n_nodes, n_offsets, offsets, n_g);
+    if (st != KDGRT_OK) return st;
+    HIP_TRY(hipSetDevice(ctx->device));
+    // Repack: 48-B geometry records (pos|opacity, inverse scale|k_scale,
+    // quaternion) and 192-B SH records.
+    std::vector<float4> geom(3 * n_g), shade(12 * n_g);
+    for (uint64_t i = 0; i < n_g; ++i) {
+        const kdgrt_gaussian& s = g[i];
+        float inv[3], kinv;
+        if (scales_inverted) {
+            inv[0] = s.scale[0];
+            inv[1] = s.scale[1];
+            inv[2] = s.scale[2];
+            kinv = s.k_scale;
+        } else {
+            inv[0] = 1 / s.scale[0];
+            inv[1] = 1 / s.scale[1];
+            inv[2] = 1 / s.scale[2];
+            kinv = 1 / s.k_scale;
+        }
+        geom[3 * i + 0] = make_float4(s.pos[0], s.pos[1], s.pos[2], s.opacity);
+        geom[3 * i + 1] = make_float4(inv[0], inv[1], inv[2], kinv);
+        geom[3 * i + 2] = make_float4(s.rot[0], s.rot[1], s.rot[2], s.rot[3]);
+        std::memcpy(&shade[12 * i], s.f_dc, 48 * sizeof(float));
+    }
+    auto* sc = new kdgrt_scene();
+    sc->ctx = ctx;
+    auto fail = [&](hipError_t e, const char* what) {
+        if (sc->d_nodes) hipFree(sc->d_nodes);
+        if (sc->d_offs) hipFree(sc->d_offs);
+        if (sc->d_geom) hipFree(sc->d_geom);
+        if (sc->d_shade) hipFree(sc->d_shade);
+        delete sc;
+        return set_error(e == hipErrorOutOfMemory ? KDGRT_ERR_OUT_OF_MEMORY : KDGRT_ERR_DEVICE,
+                         std::string("scene_create: ") + what + ": " + hipGetErrorString(e));
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&sc->d_nodes, 8ull * n_nodes)) != hipSuccess) return fail(e, "nodes");
+    if ((e = hipMalloc(&sc->d_offs, std::max<uint64_t>(4, 4ull * n_offsets))) != hipSuccess) return fail(e, "offsets");
+    if ((e = hipMalloc(&sc->d_geom, 48ull * n_g)) != hipSuccess) return fail(e, "geometry");
+    if ((e = hipMalloc(&sc->d_shade, 192ull * n_g)) != hipSuccess) return fail(e, "sh");
+    if ((e = hipMemcpy(sc->d_nodes, nodes, 8ull * n_nodes, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "copy");
+    if (n_offsets && (e = hipMemcpy(sc->d_offs, offsets, 4ull * n_offsets, hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(e, "copy");
+    if ((e = hipMemcpy(sc->d_geom, geom.data(), 48ull * n_g, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "copy");
+    if ((e = hipMemcpy(sc->d_shade, shade.data(), 192ull * n_g, hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(e, "copy");
+    sc->dev.nodes = (const uint2*)sc->d_nodes;
+    sc->dev.offs = (const uint32_t*)sc->d_offs;
+    sc->dev.geom = (const float4*)sc->d_geom;
+    sc->dev.shade = (const float4*)sc->d_shade;
+    for (int k = 0; k < 3; ++k) {
+        sc->dev.bmin[k] = aabb[k];
+        sc->dev.bmax[k] = aabb[3 + k];
+    }
+    sc->n_nodes = n_nodes;
+    sc->n_offsets = n_offsets;
+    sc->n_gaussians = n_g;
+    sc->bytes = kdgrt_scene_bytes_estimate(n_nodes, n_offsets, n_g);
+    *out = sc;
+    return KDGRT_OK;
+}
+
+kdgrt_status kdgrt_scene_create_from_tree(kdgrt_context* ctx, const kdgrt_tree* tree, const kdgrt_gaussian* g,
+                                          uint64_t n_g, int scales_inverted, kdgrt_scene** out) {
+    if (!tree) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "scene_create_from_tree: null tree");
+    return kdgrt_scene_create(ctx, tree->nodes.data(), (uint32_t)tree->nodes.size(), tree->offsets.data(),
+                              tree->offsets.size(), tree->aabb, g, n_g, scales_inverted, out);
+}
+
+kdgrt_status kdgrt_scene_destroy(kdgrt_scene* sc) {
+    if (!sc) return KDGRT_OK;
+    hipSetDevice(sc->ctx->device);
+    hipFree(sc->d_nodes);
+    hipFree(sc->d_offs);
+    hipFree(sc->d_geom);
+    hipFree(sc->d_shade);
+    delete sc;
+    return KDGRT_OK;
+}
+
+kdgrt_status kdgrt_scene_device_bytes(const kdgrt_scene* sc, uint64_t* bytes) {
+    if (!sc || !bytes) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "scene_device_bytes: null");
+    *bytes = sc->bytes;
+    return KDGRT_OK;
+}
+
+static kdgrt_status check_view(const kdgrt_context* ctx, const kdgrt_scene* sc, const kdgrt_camera_info* cam, int W,
+                               int H, int depth, const void* out) {
+    if (!ctx || !sc || !cam || !out) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render: null argument");
+    if (sc->ctx != ctx) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render: scene belongs to another context");
+    if (W <= 0 || H <= 0 || W > 32768 || H > 32768) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render: bad size");
+    if (depth < 1 || depth > 32) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render: stack depth must be 1..32");
+    return KDGRT_OK;
+}
+
+static void fill_view_args(ViewArgs& a, const kdgrt_scene* sc, const kdgrt_camera_info* cam, int W, int H,
+                           int depth, void* out) {
+    a.sc = sc->dev;
+    std::memcpy(&a.cam, cam, sizeof(CamInfo));
+    a.W = W;
+    a.H = H;
+    a.depth = depth;
+    a.blocks_x = (W + 15) / 16;
+    a.blocks_y = (H + 15) / 16;
+    a.out = out;
+    a.counters = nullptr;
+    a.hashes = nullptr;
+    a.seq = nullptr;
+    a.seq_cap = 0;
+}
+
+kdgrt_status kdgrt_render(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgrt_camera_info* cam, int W, int H,
+                          int depth, int fmt, void* out, void* stream) {
+    kdgrt_status st = check_view(ctx, sc, cam, W, H, depth, out);
+    if (st != KDGRT_OK) return st;
+    if (fmt != KDGRT_OUT_RGBT_F32 && fmt != KDGRT_OUT_RGB_F32)
+        return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render: unknown output format");
+    ViewArgs a;
+    fill_view_args(a, sc, cam, W, H, depth, out);
+    const dim3 grid(a.blocks_x * a.blocks_y), block(256);
+    hipStream_t s = (hipStream_t)stream;
+    if (depth <= 12) {
+        if (fmt == KDGRT_OUT_RGBT_F32) hipLaunchKernelGGL((k_view<12, 8, false, KDGRT_OUT_RGBT_F32>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((k_view<12, 8, false, KDGRT_OUT_RGB_F32>), grid, block, 0, s, a);
+    } else {
+        if (fmt == KDGRT_OUT_RGBT_F32) hipLaunchKernelGGL((k_view<32, 8, false, KDGRT_OUT_RGBT_F32>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((k_view<32, 8, false, KDGRT_OUT_RGB_F32>), grid, block, 0, s, a);
+    }
+    HIP_TRY(hipGetLastError());
+    return KDGRT_OK;
+}
+
+kdgrt_status kdgrt_render_debug(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgrt_camera_info* cam, int W,
+                                int H, int depth, float* out, int32_t* counters, uint64_t* hashes, uint32_t* seq,
+                                int seq_cap, void* stream) {
+    kdgrt_status st = check_view(ctx, sc, cam, W, H, depth, out);
+    if (st != KDGRT_OK) return st;
+    if (seq && seq_cap <= 0) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render_debug: seq_cap must be > 0");
+    ViewArgs a;
+    fill_view_args(a, sc, cam, W, H, depth, out);
+    a.counters = counters;
+    a.hashes = hashes;
+    a.seq = seq;
+    a.seq_cap = seq ? seq_cap : 0;
+    const dim3 grid(a.blocks_x * a.blocks_y), block(256);
+    hipStream_t s = (hipStream_t)stream;
+    if (depth <= 12) hipLaunchKernelGGL((k_view<12, 8, true, KDGRT_OUT_RGBT_F32>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_view<32, 8, true, KDGRT_OUT_RGBT_F32>), grid, block, 0, s, a);
+    HIP_TRY(hipGetLastError());
+    return KDGRT_OK;
+}
+
+static kdgrt_status launch_rays(kdgrt_context* ctx, const kdgrt_scene* sc, const float* ro, const float* rd,
+                                uint64_t n, int depth, float* out, int32_t* counters, uint64_t* hashes, bool dbg,
+                                void* stream) {
+    if (!ctx || !sc || (n && (!ro || !rd || !out))) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render_rays: null");
+    if (depth < 1 || depth > 32) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render_rays: stack depth must be 1..32");
+    if (n == 0) return KDGRT_OK;
+    if (n > (uint64_t)0x7FFFFFFF * 256ull) return set_error(KDGRT_ERR_CAPACITY, "render_rays: too many rays");
+    RaysArgs a{sc->dev, ro, rd, n, depth, (float4*)out, counters, hashes};
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    hipStream_t s = (hipStream_t)stream;
+    if (dbg) {
+        if (depth <= 12) hipLaunchKernelGGL((k_rays<12, 8, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((k_rays<32, 8, true>), grid, block, 0, s, a);
+    } else {
+        if (depth <= 12) hipLaunchKernelGGL((k_rays<12, 8, false>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((k_rays<32, 8, false>), grid, block, 0, s, a);
+    }
+    HIP_TRY(hipGetLastError());
+    return KDGRT_OK;
+}
+
+kdgrt_status kdgrt_render_rays(kdgrt_context* ctx, const kdgrt_scene* sc, const float* ro, const float* rd,
+                               uint64_t n, int depth, float* out, void* stream) {
+    return launch_rays(ctx, sc, ro, rd, n, depth, out, nullptr, nullptr, false, stream);
+}
+
+kdgrt_status kdgrt_render_rays_debug(kdgrt_context* ctx, const kdgrt_scene* sc, const float* ro, const float* rd,
+                                     uint64_t n, int depth, float* out, int32_t* counters, uint64_t* hashes,
+                                     void* stream) {
+    return launch_rays(ctx, sc, ro, rd, n, depth, out, counters, hashes, true, stream);
+}
+
+kdgrt_status kdgrt_warm_up(kdgrt_context* ctx, const kdgrt_scene* sc, void* stream) {
+    // warmUp (cudaRenderer.cu:2532-2554) renders one frame to load the code
+    // object and fault in the scene; here a 64x64 view from the box corner.
+    if (!ctx || !sc) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "warm_up: null");
+    void* tmp = nullptr;
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipMalloc(&tmp, 64 * 64 * 16));
+    kdgrt_camera cam{};
+    for (int k = 0; k < 3; ++k) cam.pos[k] = sc->dev.bmax[k] + 1.0f;
+    cam.u[0] = 1;
+    cam.v[1] = 1;
+    cam.n[2] = 1;
+    cam.fovy_deg = 60;
+    cam.aspect = 1;
+    cam.near_c = 0.005;
+    kdgrt_camera_info ci;
+    kdgrt_camera_make(&cam, 64, 64, &ci);
+    kdgrt_status st = kdgrt_render(ctx, sc, &ci, 64, 64, 12, KDGRT_OUT_RGBT_F32, tmp, stream);
+    hipStreamSynchronize((hipStream_t)stream);
+    hipFree(tmp);
+    return st;
+}
+
+kdgrt_status kdgrt_tuning_default(kdgrt_context* ctx, kdgrt_tuning* t) {
+    if (!t) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "tuning_default: null");
+    (void)ctx;
+    t->tile_w = 8;
+    t->tile_h = 8;
+    t->stack_depth = 12;
+    t->hit_capacity = 8;
+    t->waves_per_cu = 16;
+    t->block_threads = 256;
+    return KDGRT_OK;
+}
+
+kdgrt_status kdgrt_batch_create(kdgrt_context* ctx, const kdgrt_view* views, int n_views,
+                                const kdgrt_ray_set* sets, int n_sets, const kdgrt_tuning* tuning,
+                                kdgrt_batch** out) {
+    return kdgrt::batch_build(ctx, views, n_views, sets, n_sets, tuning, out);
+}
+
+kdgrt_status kdgrt_batch_run(kdgrt_batch* b, void* stream, int use_graph) {
+    if (!b) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "batch_run: null");
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    if (!use_graph) return kdgrt::batch_launch(b, s);
+    if (b->graph_exec && b->graph_stream != stream) {
+        hipGraphExecDestroy(b->graph_exec);
+        hipGraphDestroy(b->graph);
+        b->graph_exec = nullptr;
+        b->graph = nullptr;
+    }
+    if (!b->graph_exec) {
+        if (s == nullptr) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "batch_run: graph capture needs a stream");
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        kdgrt_status st = kdgrt::batch_launch(b, s);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &g);
+        if (st != KDGRT_OK) {
+            if (g) hipGraphDestroy(g);
+            return st;
+        }
+        if (e != hipSuccess) return set_error(KDGRT_ERR_DEVICE, std::string("batch_run: capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            hipGraphDestroy(g);
+            return set_error(KDGRT_ERR_DEVICE, std::string("batch_run: instantiate: ") + hipGetErrorString(e));
+        }
+        b->graph = g;
+        b->graph_stream = stream;
+    }
+    HIP_TRY(hipGraphLaunch(b->graph_exec, s));
+    return KDGRT_OK;
+}
+
+kdgrt_status kdgrt_batch_info(const kdgrt_batch* b, uint64_t* n_rays, uint32_t* n_tiles, uint32_t* grid_blocks) {
+    if (!b) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "batch_info: null");
+    if (n_rays) *n_rays = b->n_rays;
+    if (n_tiles) *n_tiles = b->n_tiles;
+    if (grid_blocks) *grid_blocks = (uint32_t)b->grid;
+    return KDGRT_OK;
+}
+
+kdgrt_status kdgrt_batch_destroy(kdgrt_batch* b) {
+    if (!b) return KDGRT_OK;
+    hipSetDevice(b->ctx->device);
+    if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    hipFree(b->d_items);
+    hipFree(b->d_counters);
+    delete b;
+    return KDGRT_OK;
+}
+
+} // extern "C"
+
+namespace kdgrt {
+
+kdgrt_status batch_build(kdgrt_context* ctx, const kdgrt_view* views, int n_views, const kdgrt_ray_set* sets,
+                         int n_sets, const kdgrt_tuning* tuning, kdgrt_batch** out) {
+    if (!ctx || !out || (n_views && !views) || (n_sets && !sets) || n_views < 0 || n_sets < 0)
+        return set_error(KDGRT_ERR_INVALID_ARGUMENT, "batch_create: null argument");
+    if (n_views + n_sets == 0) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "batch_create: empty batch");
+    kdgrt_tuning t;
+    if (tuning) t = *tuning;
+    else kdgrt_tuning_default(ctx, &t);
+    if (t.tile_w < 8 || t.tile_h < 8 || t.tile_w % 8 || t.tile_h % 8 || t.tile_w * t.tile_h > 4096)
+        return set_error(KDGRT_ERR_INVALID_ARGUMENT, "batch_create: tile must be multiples of 8, at most 4096 px");
+    if (t.stack_depth < 1 || t.stack_depth > 32)
+        return set_error(KDGRT_ERR_INVALID_ARGUMENT, "batch_create: stack depth must be 1..32");
+    if (t.hit_capacity != 4 && t.hit_capacity != 8 && t.hit_capacity != 16)
+        return set_error(KDGRT_ERR_INVALID_ARGUMENT, "batch_create: hit capacity must be 4, 8 or 16");
+    if (t.block_threads != 64 && t.block_threads != 128 && t.block_threads != 256)
+        return set_error(KDGRT_ERR_INVALID_ARGUMENT, "batch_create: block threads must be 64, 128 or 256");
+    if (t.waves_per_cu < 1 || t.waves_per_cu > 64)
+        return set_error(KDGRT_ERR_INVALID_ARGUMENT, "batch_create: waves per CU must be 1..64");
+
+    std::vector<WorkItem> items;
+    items.reserve(n_views + n_sets);
+    uint64_t tiles = 0, rays = 0;
+    const uint32_t tile_px = (uint32_t)(t.tile_w * t.tile_h);
+    for (int i = 0; i < n_views; ++i) {
+        const kdgrt_view& v = views[i];
+        if (!v.scene || !v.out_rgbt || v.width <= 0 || v.height <= 0 || v.width > 32768 || v.height > 32768)
+            return set_error(KDGRT_ERR_INVALID_ARGUMENT, "batch_create: bad view " + std::to_string(i));
+        if (v.scene->ctx != ctx) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "batch_create: foreign scene");
+        WorkItem w{};
+        w.sc = v.scene->dev;
+        std::memcpy(&w.cam, &v.cam, sizeof(CamInfo));
+        w.W = v.width;
+        w.H = v.height;
+        w.tiles_x = (v.width + t.tile_w - 1) / t.tile_w;
+        const uint32_t ty = (uint32_t)((v.height + t.tile_h - 1) / t.tile_h);
+        w.tile_begin = (uint32_t)tiles;
+        w.tile_count = (uint32_t)w.tiles_x * ty;
+        w.out = (float4*)v.out_rgbt;
+        tiles += w.tile_count;
+        rays += (uint64_t)v.width * v.height;
+        items.push_back(w);
+    }
+    for (int i = 0; i < n_sets; ++i) {
+        const kdgrt_ray_set& r = sets[i];
+        if (!r.scene || (r.n_rays && (!r.origins || !r.dirs || !r.out_rgbt)))
+            return set_error(KDGRT_ERR_INVALID_ARGUMENT, "batch_create: bad ray set " + std::to_string(i));
+        if (r.scene->ctx != ctx) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "batch_create: foreign scene");
+        if (r.n_rays == 0) continue;
+        WorkItem w{};
+        w.sc = r.scene->dev;
+        w.W = 0;
+        w.tile_begin = (uint32_t)tiles;
+        const uint64_t tc = (r.n_rays + tile_px - 1) / tile_px;
+        w.tile_count = (uint32_t)tc;
+        w.out = (float4*)r.out_rgbt;
+        w.ro = r.origins;
+        w.rd = r.dirs;
+        w.n_rays = r.n_rays;
+        tiles += tc;
+        rays += r.n_rays;
+        items.push_back(w);
+    }
+    if (tiles >= 0xFFFFFFF0ull) return set_error(KDGRT_ERR_CAPACITY, "batch_create: too many tiles");
+    if (items.empty()) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "batch_create: no rays");
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto* b = new kdgrt_batch();
+    b->ctx = ctx;
+    b->tuning = t;
+    b->n_tiles = (uint32_t)tiles;
+    b->n_rays = rays;
+    hipError_t e = hipMalloc(&b->d_items, items.size() * sizeof(WorkItem));
+    if (e == hipSuccess) e = hipMalloc(&b->d_counters, 64 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemcpy(b->d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(b->d_counters, 0, 64 * sizeof(uint32_t));
+    if (e != hipSuccess) {
+        hipFree(b->d_items);
+        hipFree(b->d_counters);
+        delete b;
+        return set_error(KDGRT_ERR_DEVICE, std::string("batch_create: ") + hipGetErrorString(e));
+    }
+    BatchArgs& a = b->args;
+    a.items = b->d_items;
+    a.n_items = (int)items.size();
+    for (int x = 0; x <= kXcds; ++x) a.q_begin[x] = (uint32_t)((uint64_t)tiles * x / kXcds);
+    a.counters = b->d_counters;
+    a.depth = t.stack_depth;
+    a.tile_w = t.tile_w;
+    a.tile_h = t.tile_h;
+    b->kernel = pick_batch_kernel(t.stack_depth <= 12 ? 12 : 32, t.hit_capacity);
+    b->block = t.block_threads;
+    const int n_cu = ctx->prop.multiProcessorCount > 0 ? ctx->prop.multiProcessorCount : 256;
+    const int waves = t.waves_per_cu * n_cu;
+    b->grid = std::max(1, waves * 64 / t.block_threads);
+    // never launch more waves than tiles
+    const int max_blocks = (int)std::max<uint64_t>(1, (tiles * 64 + t.block_threads - 1) / t.block_threads);
+    b->grid = std::min(b->grid, max_blocks);
+    *out = b;
+    return KDGRT_OK;
+}
+
+kdgrt_status batch_launch(kdgrt_batch* b, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, b->d_counters, kXcds);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(b->kernel, dim3(b->grid), dim3(b->block), 0, s, b->args);
+    HIP_TRY(hipGetLastError());
+    return KDGRT_OK;
+}
+
+} // namespace kdgrt

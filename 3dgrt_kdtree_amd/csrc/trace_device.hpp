@@ -1,0 +1,368 @@
+// Per-ray device path for gfx950: short-stack kd-tree traversal, per-leaf
+// ray/Gaussian tests, depth-ordered front-to-back compositing with early
+// termination.  One ray per lane of a 64-wide wavefront.
+//
+// Semantics follow cudaRenderer.cu (paths relative to
+// Kd-treeConverter_v0.9/Kd-treeConverter/):
+//   ray generation            :2333-2339
+//   scene slab test           :379-399, 1536-1537
+//   traversal + ring stack    :1542-1573, 1690-1708, 189-210
+//   rayPrimIntersect (volume) :1069-1135
+//   sortHits (stable by t)    :415-426
+//   composite loop            :1608-1688, particleResponse<4> :681-685,
+//                              eval_sh_final(3) :487-558
+// with the documented substitutions __fdividef -> '/', rsqrtf -> 1/sqrtf,
+// expf -> kd_expf, no FMA contraction (-ffp-contract=off).
+//
+// Machine plan (not the reference's):
+//  * the short stack is a register shift-register of DMAX entries holding
+//    the newest `depth` entries: push drops the oldest exactly like the
+//    reference's overwrite-when-full ring, pop returns the newest;
+//  * the per-leaf hit list is a fixed-capacity register buffer of K records
+//    kept sorted by (t, leaf slot), which is the stable insertion-sort
+//    order; a leaf with more than K hits is finished by exact re-scans that
+//    select the next K records above the last composited key, so any leaf
+//    size is handled with no scratch memory;
+//  * the composite step reuses the test's grayDist (bit-identical to the
+//    reference's recomputation: same operands, normalize == safe_normalize
+//    for non-zero length, and zero length is rejected by the t test) and
+//    the opacity fetched with the geometry, so a rejected sample reads no
+//    SH data and an accepted one reads 192 B;
+//  * the 15 SH basis scalars depend only on the ray direction and are
+//    computed once per ray with the reference's double-precision products.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kd_expf.h"
+
+namespace kdgrt_dev {
+
+struct DevScene {
+    const uint2* __restrict__ nodes;
+    const uint32_t* __restrict__ offs;
+    const float4* __restrict__ geom;  // 3 x float4 per Gaussian: pos|opacity, inv scale|k_scale, quat wxyz
+    const float4* __restrict__ shade; // 12 x float4 per Gaussian: f_dc[3], f_rest[45]
+    float bmin[3];
+    float bmax[3];
+};
+
+struct CamInfo {
+    float eye[3], u[3], v[3], start[3];
+    float step_x, step_y;
+};
+
+struct RayDbg {
+    int32_t cnt[8];
+    uint64_t hash;
+    uint32_t* seq;
+    int seq_cap;
+    int nseq;
+};
+
+// A validated tree visits each leaf at most once per ray; this cap is a
+// guard against malformed input only and is never reached otherwise.
+constexpr int kLeafGuard = 1 << 22;
+
+__device__ __forceinline__ void make_ray(const CamInfo& c, int x, int y, float o[3], float d[3]) {
+    const float sx = (float)x + 0.5f, sy = (float)y + 0.5f;
+    float w[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float a = c.start[k] + (c.u[k] * sx) * c.step_x;
+        float b = (c.v[k] * sy) * c.step_y;
+        w[k] = (a - b) - c.eye[k];
+        o[k] = c.eye[k];
+    }
+    const float inv = 1.0f / sqrtf(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) d[k] = w[k] * inv;
+}
+
+// SH basis scalars of eval_sh_final (cudaRenderer.cu:517-553): the double
+// constant times float operands is a double product, converted to float when
+// it meets the float3 coefficient.
+__device__ __forceinline__ void sh_basis(const float d[3], float s[16]) {
+    const double C1 = 0.4886025119029199, C2_0 = 1.0925484305920792, C2_1 = -1.0925484305920792,
+                 C2_2 = 0.31539156525252005, C2_3 = -1.0925484305920792, C2_4 = 0.5462742152960396,
+                 C3_0 = -0.5900435899266435, C3_1 = 2.890611442640554, C3_2 = -0.4570457994644658,
+                 C3_3 = 0.3731763325901154, C3_4 = -0.4570457994644658, C3_5 = 1.445305721320277,
+                 C3_6 = -0.5900435899266435;
+    const float x = d[0], y = d[1], z = d[2];
+    s[0] = 0.28209479177387814f;
+    s[1] = (float)(C1 * (double)y);
+    s[2] = (float)(C1 * (double)z);
+    s[3] = (float)(C1 * (double)x);
+    const float xx = x * x, yy = y * y, zz = z * z;
+    const float xy = x * y, yz = y * z, xz = x * z;
+    s[4] = (float)(C2_0 * (double)xy);
+    s[5] = (float)(C2_1 * (double)yz);
+    s[6] = (float)(C2_2 * (double)(2.0f * zz - xx - yy));
+    s[7] = (float)(C2_3 * (double)xz);
+    s[8] = (float)(C2_4 * (double)(xx - yy));
+    s[9] = (float)((C3_0 * (double)y) * (double)(3.0f * xx - yy));
+    s[10] = (float)((C3_1 * (double)xy) * (double)z);
+    s[11] = (float)((C3_2 * (double)y) * (double)(4.0f * zz - xx - yy));
+    s[12] = (float)((C3_3 * (double)z) * (double)(2.0f * zz - 3.0f * xx - 3.0f * yy));
+    s[13] = (float)((C3_4 * (double)x) * (double)(4.0f * zz - xx - yy));
+    s[14] = (float)((C3_5 * (double)z) * (double)(xx - yy));
+    s[15] = (float)((C3_6 * (double)x) * (double)(xx - 3.0f * yy));
+}
+
+// Radiance of one Gaussian: the 16 coefficient float3s times the basis
+// scalars, accumulated in the reference's left-to-right order with the signs
+// of degree 1 (minus, plus, minus), +0.5, clamped below at 0.
+__device__ __forceinline__ void sh_radiance(const float4* __restrict__ sh, const float s[16], float rad[3]) {
+    float c[48];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) {
+        const float4 q = sh[i];
+        c[4 * i + 0] = q.x;
+        c[4 * i + 1] = q.y;
+        c[4 * i + 2] = q.z;
+        c[4 * i + 3] = q.w;
+    }
+    // c[0..2] = f_dc, c[3 + 15*ch + i] = f_rest[15*ch + i]
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        float r = c[ch] * s[0];
+        r = r - c[3 + 15 * ch + 0] * s[1];
+        r = r + c[3 + 15 * ch + 1] * s[2];
+        r = r - c[3 + 15 * ch + 2] * s[3];
+#pragma unroll
+        for (int i = 3; i < 15; ++i) r = r + c[3 + 15 * ch + i] * s[i + 1];
+        r = r + 0.5f;
+        rad[ch] = fmaxf(r, 0.0f);
+    }
+}
+
+struct TestOut {
+    float t;
+    float gd;
+    float op;
+    bool hit;
+};
+
+// rayPrimIntersect (cudaRenderer.cu:1069-1135) on the 48-byte geometry record.
+__device__ __forceinline__ TestOut ray_gaussian(const float o[3], const float d[3], const float4* __restrict__ g3,
+                                                float t_near, float t_far) {
+    TestOut r;
+    const float4 a = g3[0], b = g3[1], q = g3[2];
+    const float giscl[3] = {b.x, b.y, b.z};
+    // quaternionWXYZToMatrixTranspose (cudaRenderer.cu:808-828)
+    const float qr = q.x, qx = q.y, qy = q.z, qz = q.w;
+    const float xx = qx * qx, yy = qy * qy, zz = qz * qz;
+    const float xy = qx * qy, xz = qx * qz, yz = qy * qz;
+    const float rx = qr * qx, ry = qr * qy, rz = qr * qz;
+    const float m[3][3] = {{(1.f - 2.f * (yy + zz)), 2.f * (xy + rz), 2.f * (xz - ry)},
+                           {2.f * (xy - rz), (1.f - 2.f * (xx + zz)), 2.f * (yz + rx)},
+                           {2.f * (xz + ry), 2.f * (yz - rx), (1.f - 2.f * (xx + yy))}};
+    const float gposc[3] = {o[0] - a.x, o[1] - a.y, o[2] - a.z};
+    float gro[3], grdu[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float pr = m[k][0] * gposc[0] + m[k][1] * gposc[1] + m[k][2] * gposc[2];
+        const float dr = m[k][0] * d[0] + m[k][1] * d[1] + m[k][2] * d[2];
+        gro[k] = giscl[k] * pr;
+        grdu[k] = dr * giscl[k];
+    }
+    const float l = grdu[0] * grdu[0] + grdu[1] * grdu[1] + grdu[2] * grdu[2];
+    float grd[3];
+    if (l > 0.0f) {
+        const float inv = 1.0f / sqrtf(l);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) grd[k] = grdu[k] * inv;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) grd[k] = grdu[k];
+    }
+    const float grp = -(grd[0] * gro[0] + grd[1] * gro[1] + grd[2] * gro[2]);
+    float grds[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) grds[k] = ((1.0f / giscl[k]) * grd[k]) * grp;
+    const float t = (grp < 0.f ? -1.f : 1.f) * sqrtf(grds[0] * grds[0] + grds[1] * grds[1] + grds[2] * grds[2]);
+    r.t = t;
+    r.op = a.w;
+    r.hit = false;
+    r.gd = 0.0f;
+    if (t < t_near || t > t_far) return r;
+    const float c0 = grd[1] * gro[2] - grd[2] * gro[1];
+    const float c1 = grd[2] * gro[0] - grd[0] * gro[2];
+    const float c2 = grd[0] * gro[1] - grd[1] * gro[0];
+    const float gd = c0 * c0 + c1 * c1 + c2 * c2;
+    r.gd = gd;
+    r.hit = gd < 8.f;
+    return r;
+}
+
+template <int DMAX, int K, bool DBG>
+__device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], const float d[3], int depth,
+                                          float rgb[3], float& A, RayDbg* dbg) {
+    rgb[0] = rgb[1] = rgb[2] = 0.0f;
+    A = 0.0f;
+    float t_near = 1e-2f, t_far = 3.402823466e+38f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float l1 = (sc.bmin[k] - o[k]) / d[k];
+        const float l2 = (sc.bmax[k] - o[k]) / d[k];
+        t_near = fmaxf(fminf(l1, l2), t_near);
+        t_far = fminf(fmaxf(l1, l2), t_far);
+    }
+    if (!((t_far >= t_near) & (t_far >= 0.f))) return;
+    const float t_scene_far = t_far;
+
+    float shs[16];
+    sh_basis(d, shs);
+
+    uint32_t stn[DMAX];
+    float stt[DMAX];
+#pragma unroll
+    for (int i = 0; i < DMAX; ++i) {
+        stn[i] = 0;
+        stt[i] = 0.f;
+    }
+    int quant = 0;
+
+    uint2 node = sc.nodes[0];
+    int guard = 0;
+    while (A < 0.99f) {
+        while ((node.x & 7u) != 3u) {
+            const uint32_t child = node.x >> 3;
+            const uint32_t axis = node.x & 3u;
+            const float oa = axis == 0 ? o[0] : (axis == 1 ? o[1] : o[2]);
+            const float da = axis == 0 ? d[0] : (axis == 1 ? d[1] : d[2]);
+            const float t_split = (__uint_as_float(node.y) - oa) / da;
+            const uint32_t sign = __float_as_uint(da) >> 31;
+            const uint32_t idx = child + (sign ^ (t_split <= t_near ? 1u : 0u));
+            if (t_near < t_split && t_split < t_far) {
+#pragma unroll
+                for (int i = DMAX - 1; i > 0; --i) {
+                    stn[i] = stn[i - 1];
+                    stt[i] = stt[i - 1];
+                }
+                stn[0] = child + (sign ^ 1u);
+                stt[0] = t_far;
+                quant = quant + 1 < depth ? quant + 1 : depth;
+                t_far = t_split;
+            }
+            if (DBG) dbg->cnt[0]++;
+            node = sc.nodes[idx];
+        }
+
+        // ---- leaf ----
+        const uint32_t base = node.y, count = node.x >> 3;
+        if (DBG) {
+            dbg->cnt[1]++;
+            dbg->cnt[2] += (int32_t)count;
+        }
+        bool have_lb = false, done = false;
+        float lb_t = 0.f;
+        uint32_t lb_s = 0;
+        int first_hits = 0;
+        while (true) {
+            float kt[K], kg[K], ko[K];
+            uint32_t ks[K];
+#pragma unroll
+            for (int i = 0; i < K; ++i) {
+                kt[i] = 0.f;
+                kg[i] = 0.f;
+                ko[i] = 0.f;
+                ks[i] = 0;
+            }
+            int n = 0;
+            bool overflow = false;
+            for (uint32_t s = 0; s < count; ++s) {
+                const uint32_t gid = sc.offs[base + s];
+                const TestOut r = ray_gaussian(o, d, sc.geom + 3 * (size_t)gid, t_near, t_far);
+                if (!r.hit) continue;
+                if (have_lb && !(r.t > lb_t || (r.t == lb_t && s > lb_s))) continue;
+                if (DBG && !have_lb) first_hits++;
+                int pos = 0;
+#pragma unroll
+                for (int i = 0; i < K; ++i) pos += (i < n && kt[i] <= r.t) ? 1 : 0;
+                if (pos >= K) {
+                    overflow = true;
+                    continue;
+                }
+                if (n == K) overflow = true;
+#pragma unroll
+                for (int i = K - 1; i > 0; --i) {
+                    if (i > pos) {
+                        kt[i] = kt[i - 1];
+                        kg[i] = kg[i - 1];
+                        ko[i] = ko[i - 1];
+                        ks[i] = ks[i - 1];
+                    } else if (i == pos) {
+                        kt[i] = r.t;
+                        kg[i] = r.gd;
+                        ko[i] = r.op;
+                        ks[i] = s;
+                    }
+                }
+                if (pos == 0) {
+                    kt[0] = r.t;
+                    kg[0] = r.gd;
+                    ko[0] = r.op;
+                    ks[0] = s;
+                }
+                n = n + 1 < K ? n + 1 : K;
+            }
+            if (DBG && !have_lb) {
+                dbg->cnt[3] += first_hits;
+                if (first_hits > dbg->cnt[5]) dbg->cnt[5] = first_hits;
+            }
+#pragma unroll
+            for (int i = 0; i < K; ++i) {
+                if (i >= n || done) break;
+                if (DBG) dbg->cnt[6]++;
+                const float gd = kg[i];
+                const float gres = kd_expf(-0.0555555555556f * gd * gd);
+                const float alpha = fminf(0.99f, ko[i] * gres);
+                if (gres < 0.0113f || alpha < 1.0f / 255.0f) continue;
+                const uint32_t gid = sc.offs[base + ks[i]];
+                float c[3];
+                sh_radiance(sc.shade + 12 * (size_t)gid, shs, c);
+                const float w = 1.0f - A;
+                rgb[0] = rgb[0] + (c[0] * alpha) * w;
+                rgb[1] = rgb[1] + (c[1] * alpha) * w;
+                rgb[2] = rgb[2] + (c[2] * alpha) * w;
+                A = A + alpha * (1.0f - A);
+                if (DBG) {
+                    dbg->cnt[4]++;
+                    dbg->hash = (dbg->hash ^ (uint64_t)gid) * 0x100000001b3ull;
+                    if (dbg->seq && dbg->nseq < dbg->seq_cap) dbg->seq[dbg->nseq] = gid;
+                    dbg->nseq++;
+                }
+                if (A > 0.99f) done = true;
+            }
+            if (done || !overflow) break;
+            lb_t = kt[K - 1];
+            lb_s = ks[K - 1];
+            have_lb = true;
+        }
+
+        if ((A > 0.99f) | (t_far >= t_scene_far)) break;
+        if (quant == 0) {
+            node = sc.nodes[0];
+            t_near = t_far;
+            t_far = t_scene_far;
+            if (DBG) dbg->cnt[7]++;
+        } else {
+            const uint32_t nid = stn[0];
+            const float tm = stt[0];
+#pragma unroll
+            for (int i = 0; i < DMAX - 1; ++i) {
+                stn[i] = stn[i + 1];
+                stt[i] = stt[i + 1];
+            }
+            --quant;
+            node = sc.nodes[nid];
+            t_near = t_far;
+            t_far = tm;
+        }
+        if (++guard > kLeafGuard) break;
+    }
+}
+
+} // namespace kdgrt_dev

@@ -1,0 +1,357 @@
+/*
+ * kdgrt.h — C ABI of the MI355X (gfx950) kd-tree 3D-Gaussian ray-render path.
+ *
+ * Every entry point returns a kdgrt_status; kdgrt_last_error() gives the
+ * message of the most recent failure on the calling thread.  Nothing in the
+ * library calls exit().  Plain C types only: no STL, no torch, no HIP types
+ * in the signatures (streams are passed as opaque void*, i.e. hipStream_t).
+ *
+ * Ownership: the library owns scene device buffers (created by
+ * kdgrt_scene_create, freed by kdgrt_scene_destroy); callers own every
+ * output buffer (device pointers from kdgrt_device_alloc or any HIP
+ * allocation).  Render calls are asynchronous on the given stream.
+ *
+ * Reference interface each entry replaces (paths relative to
+ * Kd-treeConverter_v0.9/Kd-treeConverter/):
+ *   kdgrt_init             <- initCuda()                       cudaRenderer.cu:2438-2453
+ *   kdgrt_shutdown         <- cleanupCudaResources()           cudaRenderer.cu:3486-3521
+ *   kdgrt_scene_create     <- renderGaussianWithCudaSetup()    cudaRenderer.cu:2557-2703
+ *   kdgrt_camera_make      <- CameraInfo set-up in
+ *                             renderGaussianWithCudaFrame()    cudaRenderer.cu:3108-3127
+ *   kdgrt_render           <- renderGaussianWithCudaFrame()    cudaRenderer.cu:3099-3484
+ *                             (kernel renderKernelGaussian_sortNode, :2313-2390)
+ *   kdgrt_render_debug     <- HIT_AND_NODE_COUNT_DEBUG variant cudaRenderer.cu:1504-1606, 2345-2389
+ *   kdgrt_warm_up          <- warmUp()                         cudaRenderer.cu:2532-2554
+ *   kdgrt_tree_build       <- build_kd_tree_for_composite_object()  Kd-treeConverter.cpp:22-135
+ *                             + create_composite_object_from_gaussians()
+ *                                                              Kd-treeConverterMain.cpp:1925-2099
+ *   kdgrt_kdt_write/read   <- dump_kd_tree_for_composite_object / read_kd_tree_from_file
+ *                                                              Kd-treeConverter.cpp:297-343, 444-492
+ *   kdgrt_ply_read         <- loadGaussiansFromPly()           Kd-treeConverterMain.cpp:1652-1769
+ *   kdgrt_render_rays, kdgrt_batch_*, kdgrt_server_* have no single
+ *   reference counterpart: they run the same per-ray path over explicit ray
+ *   lists, many views in one persistent launch, and a serving loop.
+ */
+#ifndef KDGRT_H
+#define KDGRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KDGRT_API __attribute__((visibility("default")))
+
+typedef enum kdgrt_status {
+    KDGRT_OK = 0,
+    KDGRT_ERR_INVALID_ARGUMENT = 1,
+    KDGRT_ERR_OUT_OF_MEMORY = 2,
+    KDGRT_ERR_DEVICE = 3,
+    KDGRT_ERR_IO = 4,
+    KDGRT_ERR_FORMAT = 5,
+    KDGRT_ERR_CAPACITY = 6,   /* over a fixed capacity or memory budget */
+    KDGRT_ERR_NOT_READY = 7,  /* asynchronous result not yet available */
+    KDGRT_ERR_NO_DEVICE = 8,
+    KDGRT_ERR_CANCELLED = 9,
+    KDGRT_ERR_TIMEOUT = 10
+} kdgrt_status;
+
+/* Gaussian record, 256 B: the reference layout with QUATERNION=true
+ * (Kd-treeConverter.h:319-335).  f_rest is channel-major: SH coefficient
+ * i+1 of channel c is f_rest[15*c + i] (cudaRenderer.cu:495-500). */
+typedef struct kdgrt_gaussian {
+    float pos[3];
+    float opacity;
+    float scale[3];
+    float k_scale;
+    float rot[4]; /* w, x, y, z */
+    float f_dc[3];
+    float f_rest[45];
+    float pad[4];
+} kdgrt_gaussian;
+
+/* kd-tree node, 8 B (Kd-treeConverter.h:282-285, macros :402-410):
+ * inner: x = (first_child << 3) | axis, y = bits of the f32 split position;
+ * leaf : x = (count << 3) | 3,          y = offset into the GID list. */
+typedef struct kdgrt_node {
+    uint32_t x;
+    uint32_t y;
+} kdgrt_node;
+
+/* Camera fields the render path reads (OpenGLStuffs.h:21-28). */
+typedef struct kdgrt_camera {
+    float pos[3];
+    float u[3];
+    float v[3];
+    float n[3]; /* forward direction is -n */
+    double fovy_deg;
+    double aspect;
+    double near_c;
+} kdgrt_camera;
+
+/* Per-view ray-generation constants (cudaRenderer.cu:318). */
+typedef struct kdgrt_camera_info {
+    float eye[3];
+    float u[3];
+    float v[3];
+    float start[3];
+    float step_x;
+    float step_y;
+} kdgrt_camera_info;
+
+/* Builder parameters (Kd-treeConverterMain.cpp:46-61, Kd-treeConverter.h:43-65). */
+typedef struct kdgrt_build_params {
+    float traversal_cost;   /* TRAVL_COST 1.0 */
+    float isect_cost;       /* ISCET_COST 5.0 */
+    float empty_bonus;      /* EMTPY_BONUS 0.9 */
+    uint32_t max_level;     /* MAX_LEVEL 256 */
+    uint32_t min_prims;     /* MIN_TRI 16 (unique Gaussian IDs) */
+    uint32_t force_split;   /* FORCE_SPLIT_THRESHOLD 256 (triangles); 0 = off */
+    int32_t sah_mode;       /* SAH_MODE 1 (0 and 1 supported) */
+    int32_t threads;        /* 0 = all hardware threads */
+} kdgrt_build_params;
+
+/* Statistics of a built or loaded tree. */
+typedef struct kdgrt_tree_stats {
+    uint32_t n_nodes;
+    uint32_t n_leaves;
+    uint32_t n_empty_leaves;
+    uint32_t max_depth;
+    uint32_t max_leaf_size;
+    uint64_t n_offsets;
+    uint64_t n_triangles;       /* proxy triangles fed to the builder */
+    uint64_t n_kept_gaussians;  /* Gaussians that passed the opacity cull */
+    double build_seconds;
+    double mesh_seconds;
+} kdgrt_tree_stats;
+
+typedef struct kdgrt_context kdgrt_context;
+typedef struct kdgrt_scene kdgrt_scene;
+typedef struct kdgrt_tree kdgrt_tree;
+typedef struct kdgrt_batch kdgrt_batch;
+typedef struct kdgrt_server kdgrt_server;
+
+/* Output pixel formats. */
+#define KDGRT_OUT_RGBT_F32 0 /* float4 per pixel: r, g, b, T = 1 - accumulated opacity */
+#define KDGRT_OUT_RGB_F32 1  /* float3 per pixel, the reference framebuffer layout */
+
+/* Debug counters per ray (int32 x 8). */
+#define KDGRT_CNT_NODE_VISITS 0
+#define KDGRT_CNT_LEAF_VISITS 1
+#define KDGRT_CNT_ISECT_TESTS 2
+#define KDGRT_CNT_HITS_FOUND 3
+#define KDGRT_CNT_BLEND_OPS 4
+#define KDGRT_CNT_MAX_SORT 5
+#define KDGRT_CNT_N_EVAL 6
+#define KDGRT_CNT_RESTARTS 7
+#define KDGRT_N_COUNTERS 8
+
+/* ---------------- errors, device, memory ---------------- */
+KDGRT_API const char* kdgrt_last_error(void);
+KDGRT_API const char* kdgrt_version(void);
+KDGRT_API kdgrt_status kdgrt_device_count(int* count);
+KDGRT_API kdgrt_status kdgrt_init(int device, kdgrt_context** ctx);
+KDGRT_API kdgrt_status kdgrt_shutdown(kdgrt_context* ctx);
+KDGRT_API kdgrt_status kdgrt_device_alloc(kdgrt_context* ctx, size_t bytes, void** dptr);
+KDGRT_API kdgrt_status kdgrt_device_free(kdgrt_context* ctx, void* dptr);
+KDGRT_API kdgrt_status kdgrt_memcpy_h2d(kdgrt_context* ctx, void* dst, const void* src, size_t bytes);
+KDGRT_API kdgrt_status kdgrt_memcpy_d2h(kdgrt_context* ctx, void* dst, const void* src, size_t bytes);
+KDGRT_API kdgrt_status kdgrt_memset(kdgrt_context* ctx, void* dptr, int value, size_t bytes);
+KDGRT_API kdgrt_status kdgrt_stream_create(kdgrt_context* ctx, void** stream);
+KDGRT_API kdgrt_status kdgrt_stream_destroy(kdgrt_context* ctx, void* stream);
+KDGRT_API kdgrt_status kdgrt_stream_sync(kdgrt_context* ctx, void* stream);
+KDGRT_API kdgrt_status kdgrt_device_sync(kdgrt_context* ctx);
+/* Timing with HIP events recorded on `stream`: begin, then end returns ms. */
+KDGRT_API kdgrt_status kdgrt_timer_begin(kdgrt_context* ctx, void* stream);
+KDGRT_API kdgrt_status kdgrt_timer_end(kdgrt_context* ctx, void* stream, float* ms);
+KDGRT_API kdgrt_status kdgrt_device_info(kdgrt_context* ctx, char* name, size_t name_len,
+                                         int* n_cu, size_t* total_mem, size_t* lds_per_block);
+
+/* ---------------- scenes (device-resident, immutable) ---------------- */
+/* nodes/offsets/aabb as in a .kdt file; aabb = xmin,ymin,zmin,xmax,ymax,zmax.
+ * scales_inverted = 1 when `g` already holds 1/scale (the reference caller
+ * contract, Kd-treeConverterMain.cpp:3636-3642); 0 makes the library invert
+ * with the same f32 operation.  The tree is validated (child and offset
+ * ranges, leaf GIDs < n_gaussians) before anything is uploaded. */
+KDGRT_API kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uint32_t n_nodes,
+                                          const uint32_t* offsets, uint64_t n_offsets, const float aabb[6],
+                                          const kdgrt_gaussian* g, uint64_t n_gaussians, int scales_inverted,
+                                          kdgrt_scene** scene);
+KDGRT_API kdgrt_status kdgrt_scene_create_from_tree(kdgrt_context* ctx, const kdgrt_tree* tree,
+                                                    const kdgrt_gaussian* g, uint64_t n_gaussians,
+                                                    int scales_inverted, kdgrt_scene** scene);
+KDGRT_API kdgrt_status kdgrt_scene_destroy(kdgrt_scene* scene);
+KDGRT_API kdgrt_status kdgrt_scene_device_bytes(const kdgrt_scene* scene, uint64_t* bytes);
+/* Rough device footprint of a scene before creating it. */
+KDGRT_API uint64_t kdgrt_scene_bytes_estimate(uint32_t n_nodes, uint64_t n_offsets, uint64_t n_gaussians);
+
+/* ---------------- camera ---------------- */
+KDGRT_API kdgrt_status kdgrt_camera_make(const kdgrt_camera* cam, int width, int height,
+                                         kdgrt_camera_info* out);
+
+/* ---------------- plain one-view launch ---------------- */
+/* out: width*height pixels in `out_format`, row (height-1-y) holds image
+ * row y as in the reference framebuffer (cudaRenderer.cu:2367).
+ * stack_depth: short-stack depth (reference 12, 1..32). */
+KDGRT_API kdgrt_status kdgrt_render(kdgrt_context* ctx, const kdgrt_scene* scene, const kdgrt_camera_info* cam,
+                                    int width, int height, int stack_depth, int out_format, void* out,
+                                    void* stream);
+/* Same path plus per-ray counters (int32 x KDGRT_N_COUNTERS, row-major by
+ * image row y, not flipped), an order-sensitive 64-bit hash of the
+ * composited Gaussian-ID sequence (FNV-1a over the IDs), and optionally the
+ * first `seq_cap` composited IDs per ray (uint32, -1 padded). Any of
+ * counters/id_hash/id_seq may be NULL. */
+KDGRT_API kdgrt_status kdgrt_render_debug(kdgrt_context* ctx, const kdgrt_scene* scene,
+                                          const kdgrt_camera_info* cam, int width, int height, int stack_depth,
+                                          float* out_rgbt, int32_t* counters, uint64_t* id_hash,
+                                          uint32_t* id_seq, int seq_cap, void* stream);
+/* Explicit rays (device float3 arrays, directions used as given). */
+KDGRT_API kdgrt_status kdgrt_render_rays(kdgrt_context* ctx, const kdgrt_scene* scene, const float* origins,
+                                         const float* dirs, uint64_t n_rays, int stack_depth, float* out_rgbt,
+                                         void* stream);
+KDGRT_API kdgrt_status kdgrt_render_rays_debug(kdgrt_context* ctx, const kdgrt_scene* scene,
+                                               const float* origins, const float* dirs, uint64_t n_rays,
+                                               int stack_depth, float* out_rgbt, int32_t* counters,
+                                               uint64_t* id_hash, void* stream);
+KDGRT_API kdgrt_status kdgrt_warm_up(kdgrt_context* ctx, const kdgrt_scene* scene, void* stream);
+
+/* ---------------- batched persistent launch ---------------- */
+typedef struct kdgrt_view {
+    const kdgrt_scene* scene;
+    kdgrt_camera_info cam;
+    int32_t width;
+    int32_t height;
+    float* out_rgbt; /* device, width*height float4, reference row flip */
+} kdgrt_view;
+
+typedef struct kdgrt_ray_set {
+    const kdgrt_scene* scene;
+    const float* origins; /* device float3[n] */
+    const float* dirs;    /* device float3[n] */
+    uint64_t n_rays;
+    float* out_rgbt;      /* device float4[n] */
+} kdgrt_ray_set;
+
+typedef struct kdgrt_tuning {
+    int32_t tile_w;          /* ray tile width in pixels (tile = one work-queue item) */
+    int32_t tile_h;
+    int32_t stack_depth;     /* 1..32 */
+    int32_t hit_capacity;    /* per-ray sorted hit buffer: 4, 8 or 16 */
+    int32_t waves_per_cu;    /* persistent grid = waves_per_cu * CUs * 64 lanes / block size */
+    int32_t block_threads;   /* 64, 128 or 256 */
+} kdgrt_tuning;
+
+KDGRT_API kdgrt_status kdgrt_tuning_default(kdgrt_context* ctx, kdgrt_tuning* t);
+/* Plan a batch: uploads the work table; the views' output buffers must stay
+ * valid while the batch is run. */
+KDGRT_API kdgrt_status kdgrt_batch_create(kdgrt_context* ctx, const kdgrt_view* views, int n_views,
+                                          const kdgrt_ray_set* ray_sets, int n_ray_sets,
+                                          const kdgrt_tuning* tuning, kdgrt_batch** batch);
+/* One persistent launch over all tiles of the batch on `stream`.
+ * use_graph = 1 captures reset+launch into a hipGraph on first use and
+ * replays it afterwards. */
+KDGRT_API kdgrt_status kdgrt_batch_run(kdgrt_batch* batch, void* stream, int use_graph);
+KDGRT_API kdgrt_status kdgrt_batch_info(const kdgrt_batch* batch, uint64_t* n_rays, uint32_t* n_tiles,
+                                        uint32_t* grid_blocks);
+KDGRT_API kdgrt_status kdgrt_batch_destroy(kdgrt_batch* batch);
+/* Autotune over tile size, stack depth, hit capacity, waves/CU and block
+ * size on the given batch's work; writes the fastest tuning to `best`
+ * and its mean time in ms to `best_ms`.  max_trials bounds the search. */
+KDGRT_API kdgrt_status kdgrt_autotune(kdgrt_context* ctx, const kdgrt_view* views, int n_views,
+                                      const kdgrt_ray_set* ray_sets, int n_ray_sets, int max_trials,
+                                      int reps, kdgrt_tuning* best, float* best_ms);
+
+/* ---------------- serving loop ---------------- */
+typedef struct kdgrt_server_config {
+    uint64_t hbm_budget_bytes;   /* scenes + in-flight request buffers */
+    int32_t max_batch_views;     /* views per persistent launch */
+    int32_t max_inflight_batches;/* launches overlapping on the device */
+    int32_t streams;             /* HIP streams used for overlap */
+    kdgrt_tuning tuning;
+    double batch_window_ms;      /* wait this long for more arrivals once one is queued */
+} kdgrt_server_config;
+
+typedef struct kdgrt_request {
+    int32_t scene_id;
+    kdgrt_camera_info cam;
+    int32_t width;
+    int32_t height;
+    double arrival_s;            /* caller clock; 0 = now */
+    double deadline_s;           /* 0 = none; a request not started by then is cancelled */
+} kdgrt_request;
+
+typedef struct kdgrt_result_info {
+    uint64_t request_id;
+    int32_t status;              /* kdgrt_status */
+    double arrival_s, admitted_s, launched_s, done_s; /* server clock, seconds */
+    uint32_t batch_id;
+    uint32_t batch_views;
+} kdgrt_result_info;
+
+typedef struct kdgrt_server_stats {
+    uint64_t submitted, completed, rejected, cancelled;
+    uint64_t batches, rays;
+    double busy_s, wall_s;
+    uint64_t bytes_resident, bytes_peak;
+} kdgrt_server_stats;
+
+KDGRT_API kdgrt_status kdgrt_server_create(kdgrt_context* ctx, const kdgrt_server_config* cfg,
+                                           kdgrt_server** server);
+/* Makes a scene resident (counted against the budget). */
+KDGRT_API kdgrt_status kdgrt_server_add_scene(kdgrt_server* s, const kdgrt_tree* tree, const kdgrt_gaussian* g,
+                                              uint64_t n_gaussians, int scales_inverted, int32_t* scene_id);
+KDGRT_API kdgrt_status kdgrt_server_remove_scene(kdgrt_server* s, int32_t scene_id);
+KDGRT_API kdgrt_status kdgrt_server_start(kdgrt_server* s);
+/* Thread-safe.  Returns KDGRT_ERR_CAPACITY when the request can never fit the budget. */
+KDGRT_API kdgrt_status kdgrt_server_submit(kdgrt_server* s, const kdgrt_request* req, uint64_t* request_id);
+/* Waits up to timeout_ms for request completion; copies the RGBT image
+ * (width*height float4) to host memory `host_rgbt` if not NULL. */
+KDGRT_API kdgrt_status kdgrt_server_wait(kdgrt_server* s, uint64_t request_id, double timeout_ms,
+                                         float* host_rgbt, kdgrt_result_info* info);
+KDGRT_API kdgrt_status kdgrt_server_cancel(kdgrt_server* s, uint64_t request_id);
+KDGRT_API kdgrt_status kdgrt_server_stats_get(kdgrt_server* s, kdgrt_server_stats* st);
+KDGRT_API double kdgrt_server_now(kdgrt_server* s);
+KDGRT_API kdgrt_status kdgrt_server_stop(kdgrt_server* s);
+KDGRT_API kdgrt_status kdgrt_server_destroy(kdgrt_server* s);
+
+/* ---------------- host side: data, meshing, SAH build, files ---------------- */
+KDGRT_API kdgrt_status kdgrt_build_params_default(kdgrt_build_params* p);
+/* Proxy-meshes the Gaussians (20-triangle icosahedra, opacity cull) and
+ * builds the SAH kd-tree with Gaussian IDs in the leaves. `g` holds
+ * activated, non-inverted scales. */
+KDGRT_API kdgrt_status kdgrt_tree_build(const kdgrt_gaussian* g, uint64_t n_gaussians,
+                                        const kdgrt_build_params* params, kdgrt_tree** tree);
+KDGRT_API kdgrt_status kdgrt_tree_from_arrays(const kdgrt_node* nodes, uint32_t n_nodes, const uint32_t* offsets,
+                                              uint64_t n_offsets, const float aabb[6], kdgrt_tree** tree);
+KDGRT_API kdgrt_status kdgrt_tree_stats_get(const kdgrt_tree* tree, kdgrt_tree_stats* st);
+KDGRT_API const kdgrt_node* kdgrt_tree_nodes(const kdgrt_tree* tree);
+KDGRT_API const uint32_t* kdgrt_tree_offsets(const kdgrt_tree* tree);
+KDGRT_API kdgrt_status kdgrt_tree_aabb(const kdgrt_tree* tree, float aabb[6]);
+/* Checks child/offset ranges, leaf GID ranges (< n_gaussians when nonzero). */
+KDGRT_API kdgrt_status kdgrt_tree_validate(const kdgrt_tree* tree, uint64_t n_gaussians);
+KDGRT_API kdgrt_status kdgrt_tree_free(kdgrt_tree* tree);
+KDGRT_API kdgrt_status kdgrt_kdt_write(const kdgrt_tree* tree, const char* path);
+KDGRT_API kdgrt_status kdgrt_kdt_read(const char* path, kdgrt_tree** tree);
+
+/* Proxy mesh only (for inspection/tests): writes 9 floats per triangle and the
+ * Gaussian ID of each triangle; returns counts and the vertex AABB in
+ * .kdt order.  Pass NULL buffers to query n_triangles first. */
+KDGRT_API kdgrt_status kdgrt_proxy_mesh(const kdgrt_gaussian* g, uint64_t n_gaussians, float* tri_xyz,
+                                        uint32_t* tri_gid, uint64_t capacity, uint64_t* n_triangles,
+                                        float aabb[6]);
+
+/* Synthetic scenes.  kind 0: the 10k uniform-cube scene family;
+ * kind 1: the clustered family (64 anisotropic clusters + 10% background). */
+KDGRT_API kdgrt_status kdgrt_generate_gaussians(int kind, uint64_t n, uint64_t seed, kdgrt_gaussian* out);
+/* PLY reader (3DGS property names; activation as the reference loader). */
+KDGRT_API kdgrt_status kdgrt_ply_count(const char* path, uint64_t* n);
+KDGRT_API kdgrt_status kdgrt_ply_read(const char* path, kdgrt_gaussian* out, uint64_t capacity, uint64_t* n);
+/* In-place 1/scale and 1/k_scale (Kd-treeConverterMain.cpp:3636-3642). */
+KDGRT_API kdgrt_status kdgrt_gaussians_invert_scales(kdgrt_gaussian* g, uint64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KDGRT_H */

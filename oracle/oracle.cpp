@@ -1,0 +1,437 @@
+// ORACLE — test infrastructure only.  Nothing in the product links or calls
+// this file; only tests/, __graft_entry__.smoke() and bench.py's CPU
+// baseline leg load liboracle.so, as the checker.
+//
+// A CPU restatement of the reference's plain render path
+// (renderKernelGaussian_sortNode, SECONDARY_RAY=0, PRIMITIVE_TYPE
+// ELLIPSOID_BY_TRI, QUATERNION, VOLUME_ISECT, GAUSSIAN_DEGREE 4,
+// SPH_EVAL_DEGREE 3), written from the reference source, one ray per call,
+// templated on the arithmetic type:
+//   T = float  : the FP32 oracle.  Parity gate: the HIP kernel must match it
+//                bit for bit (composited IDs, counters) and within 1e-5 (RGB, T).
+//   T = double : the FP64 oracle, for error statistics only.
+// Paths below are relative to Kd-treeConverter_v0.9/Kd-treeConverter/.
+//
+// Substitutions for CUDA intrinsics whose bits cannot be reproduced off
+// NVIDIA hardware (the reference is built without fast-math, so '/', sqrtf
+// are IEEE there too):
+//   __fdividef(a,b) -> a / b         (cudaRenderer.cu:383-394, 1560)
+//   rsqrtf(x)       -> 1.0f/sqrtf(x) (cudaRenderer.cu:55)
+//   expf            -> kd_expf       (own Cody-Waite exp, shared with the kernel)
+//   nvcc FMA contraction -> none    (built with -ffp-contract=off)
+// Parity against NVIDIA FP32 bits is therefore "parity unpinned"; this
+// restatement is the definitional FP32 result.
+//
+// Pins (tests/): closed forms for ray generation, slab test, intersection,
+// response and SH; brute force on single-leaf scenes; hand-built trees with
+// hand-counted node visits (ring-stack overwrite and restart); FP64
+// agreement; golden fixtures.
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include <omp.h>
+
+#include "../3dgrt_kdtree_amd/csrc/kd_expf.h"
+
+namespace {
+
+struct Gauss256 { // Kd-treeConverter.h:319-335 (QUATERNION)
+    float pos[3];
+    float opacity;
+    float scale[3]; // inverted scale on upload (Kd-treeConverterMain.cpp:3636-3642)
+    float k_scale;
+    float rot[4]; // w, x, y, z
+    float f_dc[3];
+    float f_rest[45];
+    float pad[4];
+};
+static_assert(sizeof(Gauss256) == 256, "Gaussian record must be 256 B");
+
+struct Node {
+    uint32_t x, y;
+};
+
+struct CamInfo { // cudaRenderer.cu:318
+    float eye[3], u[3], v[3], start[3];
+    float step_x, step_y;
+};
+
+template <class T>
+struct V3 {
+    T x, y, z;
+};
+// SGRTx2Lib/cuda_math.h operator semantics (component-wise, left to right).
+template <class T> inline V3<T> add(V3<T> a, V3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T> inline V3<T> sub(V3<T> a, V3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T> inline V3<T> mul(V3<T> a, V3<T> b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+template <class T> inline V3<T> muls(V3<T> a, T s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class T> inline T dot(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T> inline V3<T> cross(V3<T> a, V3<T> b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class T> inline T tsqrt(T x) { return std::sqrt(x); }
+inline float tsqrt(float x) { return sqrtf(x); }
+// normalize: v * (1/sqrtf(dot(v,v)))  (cuda_math.h:403-407)
+template <class T> inline V3<T> normalize(V3<T> v) {
+    T inv = T(1) / tsqrt(dot(v, v));
+    return muls(v, inv);
+}
+// safe_normalize with rsqrtf -> 1/sqrtf (cudaRenderer.cu:53-56)
+template <class T> inline V3<T> safe_normalize(V3<T> v) {
+    const T l = v.x * v.x + v.y * v.y + v.z * v.z;
+    return l > T(0) ? muls(v, T(1) / tsqrt(l)) : v;
+}
+template <class T> inline T texp(T x) { return std::exp(x); }
+inline float texp(float x) { return kd_expf(x); }
+template <class T> inline T tmin(T a, T b) { return std::fmin(a, b); }
+template <class T> inline T tmax(T a, T b) { return std::fmax(a, b); }
+
+// float33 * p: (dot(m0,p), dot(m1,p), dot(m2,p))  (cudaRenderer.cu:49-51)
+template <class T> inline V3<T> rowmul(const V3<T> m[3], V3<T> p) { return {dot(m[0], p), dot(m[1], p), dot(m[2], p)}; }
+
+// quaternionWXYZToMatrixTranspose (cudaRenderer.cu:808-828)
+template <class T> inline void quat_to_mt(const float q[4], V3<T> m[3]) {
+    const T r = q[0], x = q[1], y = q[2], z = q[3];
+    const T xx = x * x, yy = y * y, zz = z * z;
+    const T xy = x * y, xz = x * z, yz = y * z;
+    const T rx = r * x, ry = r * y, rz = r * z;
+    m[0] = {(T(1) - T(2) * (yy + zz)), T(2) * (xy + rz), T(2) * (xz - ry)};
+    m[1] = {T(2) * (xy - rz), (T(1) - T(2) * (xx + zz)), T(2) * (yz + rx)};
+    m[2] = {T(2) * (xz + ry), T(2) * (yz - rx), (T(1) - T(2) * (xx + yy))};
+}
+
+// Spherical-harmonic constants: double literals (cudaRenderer.h:33-48).
+const double SH_C0 = 0.28209479177387814, SH_C1 = 0.4886025119029199, SH_C2_0 = 1.0925484305920792,
+             SH_C2_1 = -1.0925484305920792, SH_C2_2 = 0.31539156525252005, SH_C2_3 = -1.0925484305920792,
+             SH_C2_4 = 0.5462742152960396, SH_C3_0 = -0.5900435899266435, SH_C3_1 = 2.890611442640554,
+             SH_C3_2 = -0.4570457994644658, SH_C3_3 = 0.3731763325901154, SH_C3_4 = -0.4570457994644658,
+             SH_C3_5 = 1.445305721320277, SH_C3_6 = -0.5900435899266435;
+
+// eval_sh_final(3, dir, g) (cudaRenderer.cu:487-558).  A double constant
+// times a float scalar is a double product; it reaches the float3 operator*
+// converted to float (the only viable overload is operator*(float, float3)).
+template <class T> inline V3<T> eval_sh(V3<T> d, const Gauss256& g) {
+    V3<T> c[16];
+    c[0] = {T(g.f_dc[0]), T(g.f_dc[1]), T(g.f_dc[2])};
+    for (int i = 0; i < 15; ++i) c[i + 1] = {T(g.f_rest[i]), T(g.f_rest[15 + i]), T(g.f_rest[30 + i])};
+    auto S = [](double v) { return T(v); };
+    V3<T> rad = muls(c[0], S(SH_C0));
+    const T x = d.x, y = d.y, z = d.z;
+    rad = sub(rad, muls(c[1], S(SH_C1 * y)));
+    rad = add(rad, muls(c[2], S(SH_C1 * z)));
+    rad = sub(rad, muls(c[3], S(SH_C1 * x)));
+    const T xx = x * x, yy = y * y, zz = z * z;
+    const T xy = x * y, yz = y * z, xz = x * z;
+    rad = add(rad, muls(c[4], S(SH_C2_0 * xy)));
+    rad = add(rad, muls(c[5], S(SH_C2_1 * yz)));
+    rad = add(rad, muls(c[6], S(SH_C2_2 * (T(2) * zz - xx - yy))));
+    rad = add(rad, muls(c[7], S(SH_C2_3 * xz)));
+    rad = add(rad, muls(c[8], S(SH_C2_4 * (xx - yy))));
+    rad = add(rad, muls(c[9], S(SH_C3_0 * y * (T(3) * xx - yy))));
+    rad = add(rad, muls(c[10], S(SH_C3_1 * xy * z)));
+    rad = add(rad, muls(c[11], S(SH_C3_2 * y * (T(4) * zz - xx - yy))));
+    rad = add(rad, muls(c[12], S(SH_C3_3 * z * (T(2) * zz - T(3) * xx - T(3) * yy))));
+    rad = add(rad, muls(c[13], S(SH_C3_4 * x * (T(4) * zz - xx - yy))));
+    rad = add(rad, muls(c[14], S(SH_C3_5 * z * (xx - yy))));
+    rad = add(rad, muls(c[15], S(SH_C3_6 * x * (xx - T(3) * yy))));
+    rad = add(rad, V3<T>{T(0.5f), T(0.5f), T(0.5f)});
+    return {tmax(rad.x, T(0)), tmax(rad.y, T(0)), tmax(rad.z, T(0))};
+}
+
+struct Hit {
+    float t64_unused;
+};
+
+template <class T>
+struct HitRec {
+    T t;
+    uint32_t id;
+};
+
+struct Scene {
+    const Node* nodes;
+    const uint32_t* offsets;
+    const Gauss256* g;
+    float bmin[3], bmax[3];
+};
+
+struct RayOut {
+    int32_t cnt[8];
+    uint64_t hash;
+};
+
+const uint64_t kFnvBasis = 0xcbf29ce484222325ull, kFnvPrime = 0x100000001b3ull;
+
+// rayPrimIntersect, VOLUME_ISECT (cudaRenderer.cu:1069-1135).
+template <class T>
+inline void ray_prim(const V3<T>& o, const V3<T>& d, uint32_t id, const Gauss256& g, T t_near, T t_far,
+                     std::vector<HitRec<T>>& hits) {
+    const V3<T> ppos = {T(g.pos[0]), T(g.pos[1]), T(g.pos[2])};
+    const V3<T> giscl = {T(g.scale[0]), T(g.scale[1]), T(g.scale[2])};
+    const V3<T> pscale = {T(1) / giscl.x, T(1) / giscl.y, T(1) / giscl.z};
+    V3<T> m[3];
+    quat_to_mt<T>(g.rot, m);
+    const V3<T> gposc = sub(o, ppos);
+    const V3<T> gposcr = rowmul(m, gposc);
+    const V3<T> gro = mul(giscl, gposcr);
+    const V3<T> rayDirR = rowmul(m, d);
+    const V3<T> grdu = mul(rayDirR, giscl);
+    const V3<T> grd = safe_normalize(grdu);
+    const T grp = -dot(grd, gro);
+    const V3<T> grds = muls(mul(pscale, grd), grp);
+    const T t = (grp < T(0) ? T(-1) : T(1)) * tsqrt(dot(grds, grds));
+    if (t < t_near || t > t_far) return;
+    const V3<T> gcrod = cross(grd, gro);
+    const T grayDist = dot(gcrod, gcrod);
+    if (grayDist < T(8)) hits.push_back({t, id});
+}
+
+// singlePassIntersectGaussian_sortNode_onlyShortStack, ELLIPSOID_BY_TRI
+// branch (cudaRenderer.cu:1488-1752), with the HIT_AND_NODE_COUNT_DEBUG
+// counters (:1504-1606, 1679-1681).  D = short-stack depth (ring of
+// cudaRenderer.cu:189-210; the reference uses SHORT_STACK_DEPTH 12).
+template <class T>
+void trace(const Scene& sc, V3<T> o, V3<T> d, int D, V3<T>& color, T& A, RayOut& ro, uint32_t* seq, int seq_cap) {
+    std::memset(ro.cnt, 0, sizeof(ro.cnt));
+    ro.hash = kFnvBasis;
+    int nseq = 0;
+    color = {T(0), T(0), T(0)};
+    A = T(0);
+    // BoundsRayIntersect (cudaRenderer.cu:379-399), t in [1e-2, FLT_MAX].
+    T t_scene_near = T(1e-2f), t_scene_far = T(FLT_MAX);
+    {
+        const T po[3] = {o.x, o.y, o.z}, pd[3] = {d.x, d.y, d.z};
+        for (int k = 0; k < 3; ++k) {
+            T l1 = (T(sc.bmin[k]) - po[k]) / pd[k];
+            T l2 = (T(sc.bmax[k]) - po[k]) / pd[k];
+            t_scene_near = tmax(tmin(l1, l2), t_scene_near);
+            t_scene_far = tmin(tmax(l1, l2), t_scene_far);
+        }
+        if (!((t_scene_far >= t_scene_near) & (t_scene_far >= T(0)))) return;
+    }
+    // Ring stack (cudaRenderer.cu:189-210).
+    std::vector<uint32_t> st_node(D);
+    std::vector<T> st_t(D);
+    unsigned top = (unsigned)D - 1, quant = 0;
+
+    Node node = sc.nodes[0];
+    T t_near = t_scene_near, t_far = t_scene_far;
+    std::vector<HitRec<T>> hits;
+    const T po[3] = {o.x, o.y, o.z}, pd[3] = {d.x, d.y, d.z};
+    while (A < T(0.99f)) {
+        while ((node.x & 7u) != 3u) {
+            const unsigned child = node.x >> 3;
+            const unsigned axis = node.x & 3u;
+            const unsigned ax = axis > 2 ? 2 : axis; // get_dir_pos: default -> z
+            float split;
+            std::memcpy(&split, &node.y, 4);
+            const T t_split = (T(split) - po[ax]) / pd[ax];
+            const unsigned sign = std::signbit(pd[ax]) ? 1u : 0u;
+            const unsigned idx = child + (sign ^ (t_split <= t_near ? 1u : 0u));
+            if (t_near < t_split && t_split < t_far) {
+                if (++top == (unsigned)D) top = 0;
+                quant = quant + 1 < (unsigned)D ? quant + 1 : (unsigned)D;
+                st_node[top] = child + (sign ^ 1u);
+                st_t[top] = t_far;
+                t_far = t_split;
+            }
+            ro.cnt[0]++; // node_visits
+            node = sc.nodes[idx];
+        }
+        const uint32_t base = node.y, count = node.x >> 3;
+        hits.clear();
+        ro.cnt[1]++; // leaf_visits
+        for (uint32_t k = 0; k < count; ++k) {
+            const uint32_t id = sc.offsets[base + k];
+            ray_prim<T>(o, d, id, sc.g[id], t_near, t_far, hits);
+        }
+        ro.cnt[2] += (int32_t)count;       // intersection_tests
+        ro.cnt[3] += (int32_t)hits.size(); // hits_found
+        if ((int32_t)hits.size() > ro.cnt[5]) ro.cnt[5] = (int32_t)hits.size();
+        // sortHits: stable insertion sort on t (cudaRenderer.cu:415-426).
+        for (size_t i = 1; i < hits.size(); ++i) {
+            HitRec<T> key = hits[i];
+            long j = (long)i - 1;
+            while (j >= 0 && hits[j].t > key.t) {
+                hits[j + 1] = hits[j];
+                --j;
+            }
+            hits[j + 1] = key;
+        }
+        // Composite loop (cudaRenderer.cu:1608-1688).
+        for (size_t i = 0; i < hits.size(); ++i) {
+            ro.cnt[6]++; // N_eval
+            const uint32_t gid = hits[i].id;
+            const Gauss256& g = sc.g[gid];
+            const V3<T> ppos = {T(g.pos[0]), T(g.pos[1]), T(g.pos[2])};
+            const V3<T> giscl = {T(g.scale[0]), T(g.scale[1]), T(g.scale[2])};
+            V3<T> m[3];
+            quat_to_mt<T>(g.rot, m);
+            const V3<T> gposc = sub(o, ppos);
+            const V3<T> gposcr = rowmul(m, gposc);
+            const V3<T> gro = mul(giscl, gposcr);
+            const V3<T> rayDirR = rowmul(m, d);
+            const V3<T> grdu = mul(giscl, rayDirR);
+            const V3<T> grd = normalize(grdu);
+            const V3<T> gcrod = cross(grd, gro);
+            const T grayDist = dot(gcrod, gcrod);
+            // particleResponse<4> (cudaRenderer.cu:681-685)
+            const T gres = texp(T(-0.0555555555556f) * grayDist * grayDist);
+            const T sample_opacity = tmin(T(0.99f), T(g.opacity) * gres);
+            if (gres < T(0.0113f) || sample_opacity < T(1.0f / 255.0f)) continue;
+            const V3<T> c = eval_sh<T>(d, g);
+            color = add(color, muls(muls(c, sample_opacity), T(1) - A));
+            A = A + sample_opacity * (T(1) - A);
+            ro.cnt[4]++; // blend_ops
+            ro.hash = (ro.hash ^ (uint64_t)gid) * kFnvPrime;
+            if (seq && nseq < seq_cap) seq[nseq] = gid;
+            nseq++;
+            if (A > T(0.99f)) break;
+        }
+        if ((A > T(0.99f)) | (t_far >= t_scene_far)) break;
+        if (quant == 0) {
+            node = sc.nodes[0]; // restart from the root
+            t_near = t_far;
+            t_far = t_scene_far;
+            ro.cnt[7]++;
+        } else {
+            const uint32_t nid = st_node[top];
+            const T tm = st_t[top];
+            if (top == 0) top = (unsigned)D;
+            --top;
+            --quant;
+            node = sc.nodes[nid];
+            t_near = t_far;
+            t_far = tm;
+        }
+    }
+}
+
+// Ray generation (cudaRenderer.cu:2333-2339).
+template <class T> inline void make_ray(const CamInfo& c, int x, int y, V3<T>& o, V3<T>& d) {
+    const T sx = T((float)x + 0.5f), sy = T((float)y + 0.5f);
+    const V3<T> start = {T(c.start[0]), T(c.start[1]), T(c.start[2])};
+    const V3<T> u = {T(c.u[0]), T(c.u[1]), T(c.u[2])}, v = {T(c.v[0]), T(c.v[1]), T(c.v[2])};
+    const V3<T> eye = {T(c.eye[0]), T(c.eye[1]), T(c.eye[2])};
+    const V3<T> dir = sub(add(start, muls(muls(u, sx), T(c.step_x))), muls(muls(v, sy), T(c.step_y)));
+    o = eye;
+    d = normalize(sub(dir, eye));
+}
+
+template <class T>
+void render_view(const Scene& sc, const CamInfo& cam, int W, int H, int D, T* out_rgbt, int32_t* counters,
+                 uint64_t* hashes, uint32_t* seq, int seq_cap, int threads) {
+    if (threads > 0) omp_set_num_threads(threads);
+    const int tiles_x = (W + 15) / 16, tiles_y = (H + 15) / 16;
+#pragma omp parallel for schedule(dynamic)
+    for (int tile = 0; tile < tiles_x * tiles_y; ++tile) {
+        const int tx = tile % tiles_x, ty = tile / tiles_x;
+        for (int y = ty * 16; y < std::min(H, ty * 16 + 16); ++y)
+            for (int x = tx * 16; x < std::min(W, tx * 16 + 16); ++x) {
+                V3<T> o, d, c;
+                T A;
+                RayOut ro;
+                const size_t rid = (size_t)y * W + x;
+                make_ray<T>(cam, x, y, o, d);
+                trace<T>(sc, o, d, D, c, A, ro, seq ? seq + rid * seq_cap : nullptr, seq_cap);
+                const size_t pid = (size_t)(H - y - 1) * W + x; // framebuffer row flip (cudaRenderer.cu:2367)
+                if (out_rgbt) {
+                    out_rgbt[4 * pid + 0] = c.x;
+                    out_rgbt[4 * pid + 1] = c.y;
+                    out_rgbt[4 * pid + 2] = c.z;
+                    out_rgbt[4 * pid + 3] = T(1) - A;
+                }
+                if (counters) std::memcpy(counters + 8 * rid, ro.cnt, sizeof(ro.cnt));
+                if (hashes) hashes[rid] = ro.hash;
+            }
+    }
+}
+
+template <class T>
+void render_rays(const Scene& sc, const float* origins, const float* dirs, uint64_t n, int D, T* out_rgbt,
+                 int32_t* counters, uint64_t* hashes, int threads) {
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long i = 0; i < (long long)n; ++i) {
+        const V3<T> o = {T(origins[3 * i]), T(origins[3 * i + 1]), T(origins[3 * i + 2])};
+        const V3<T> d = {T(dirs[3 * i]), T(dirs[3 * i + 1]), T(dirs[3 * i + 2])};
+        V3<T> c;
+        T A;
+        RayOut ro;
+        trace<T>(sc, o, d, D, c, A, ro, nullptr, 0);
+        if (out_rgbt) {
+            out_rgbt[4 * i + 0] = c.x;
+            out_rgbt[4 * i + 1] = c.y;
+            out_rgbt[4 * i + 2] = c.z;
+            out_rgbt[4 * i + 3] = T(1) - A;
+        }
+        if (counters) std::memcpy(counters + 8 * i, ro.cnt, sizeof(ro.cnt));
+        if (hashes) hashes[i] = ro.hash;
+    }
+}
+
+Scene make_scene(const void* nodes, const uint32_t* offsets, const void* g, const float* aabb) {
+    Scene sc;
+    sc.nodes = (const Node*)nodes;
+    sc.offsets = offsets;
+    sc.g = (const Gauss256*)g;
+    for (int k = 0; k < 3; ++k) {
+        sc.bmin[k] = aabb[k];
+        sc.bmax[k] = aabb[3 + k];
+    }
+    return sc;
+}
+
+} // namespace
+
+extern "C" {
+
+// g: 256-B Gaussian records with inverted scales; aabb in .kdt order.
+// out_rgbt: W*H*4 (row-flipped), counters: W*H*8 int32 by (y*W+x),
+// hashes: W*H uint64, seq: W*H*seq_cap uint32 (caller fills with 0xFFFFFFFF).
+void oracle_render_f32(const void* nodes, const uint32_t* offsets, const void* g, const float* aabb,
+                       const void* cam, int W, int H, int D, float* out_rgbt, int32_t* counters, uint64_t* hashes,
+                       uint32_t* seq, int seq_cap, int threads) {
+    render_view<float>(make_scene(nodes, offsets, g, aabb), *(const CamInfo*)cam, W, H, D, out_rgbt, counters,
+                       hashes, seq, seq_cap, threads);
+}
+void oracle_render_f64(const void* nodes, const uint32_t* offsets, const void* g, const float* aabb,
+                       const void* cam, int W, int H, int D, double* out_rgbt, int32_t* counters, uint64_t* hashes,
+                       int threads) {
+    render_view<double>(make_scene(nodes, offsets, g, aabb), *(const CamInfo*)cam, W, H, D, out_rgbt, counters,
+                        hashes, nullptr, 0, threads);
+}
+void oracle_render_rays_f32(const void* nodes, const uint32_t* offsets, const void* g, const float* aabb,
+                            const float* origins, const float* dirs, uint64_t n, int D, float* out_rgbt,
+                            int32_t* counters, uint64_t* hashes, int threads) {
+    render_rays<float>(make_scene(nodes, offsets, g, aabb), origins, dirs, n, D, out_rgbt, counters, hashes,
+                       threads);
+}
+void oracle_render_rays_f64(const void* nodes, const uint32_t* offsets, const void* g, const float* aabb,
+                            const float* origins, const float* dirs, uint64_t n, int D, double* out_rgbt,
+                            int32_t* counters, uint64_t* hashes, int threads) {
+    render_rays<double>(make_scene(nodes, offsets, g, aabb), origins, dirs, n, D, out_rgbt, counters, hashes,
+                        threads);
+}
+float oracle_expf(float x) { return kd_expf(x); }
+
+// Ray directions of a view (for tests of D1/D2).
+void oracle_ray_dirs(const void* cam, int W, int H, float* dirs) {
+    const CamInfo& c = *(const CamInfo*)cam;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            V3<float> o, d;
+            make_ray<float>(c, x, y, o, d);
+            float* p = dirs + 3 * ((size_t)y * W + x);
+            p[0] = d.x;
+            p[1] = d.y;
+            p[2] = d.z;
+        }
+}
+int oracle_max_threads(void) { return omp_get_max_threads(); }
+
+} // extern "C"
