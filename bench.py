@@ -1,0 +1,227 @@
+"""Benchmark of the kd-tree 3D-Gaussian ray-render hot path on one MI355X.
+
+Headline line (one JSON object on stdout): the plain one-view launch over the
+1M-Gaussian clustered scene at 1920x1080 (1 ray/pixel), Mrays/s with the
+scene resident in HBM, plus
+  roofline      algorithmic bytes per launch / mean kernel time vs 8 TB/s,
+  cpu_baseline  the FP32 oracle (CPU restatement of the reference
+                traversal+compositing, OpenMP) on this host's cores,
+  batched       the 64 mixed-resolution views in one persistent launch
+                (hipGraph replay) against 64 plain launches of the same views,
+  build         host SAH build time of the scene.
+"""
+import argparse
+import importlib
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+K = importlib.import_module("3dgrt_kdtree_amd")
+
+METRIC = "Mrays/s + HBM roofline % @1M Gaussians; p50/p99 view latency, served vs plain"
+HBM_PEAK_GBS = 8000.0
+RESOLUTIONS = [(320, 240), (640, 480), (1280, 720), (1920, 1080)]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def scene_1m(n=1_000_000, seed=1001):
+    g = K.generate_gaussians(1, n, seed)
+    t0 = time.time()
+    tree = K.Tree.build(g)
+    return g, tree, time.time() - t0
+
+
+def view_1080p(aabb):
+    centre = 0.5 * (aabb[:3].astype(np.float64) + aabb[3:].astype(np.float64))
+    d = np.array([0.48, 0.36, 0.8])
+    d /= np.linalg.norm(d)
+    cam = K.look_at(centre + 25.0 * d, centre, 60.0, 1920 / 1080, 0.005)
+    return K.camera_make(cam, 1920, 1080)
+
+
+def views_64(aabb, seed=1002):
+    """64 views: resolution uniform over the four sizes, eye uniform on the
+    sphere of radius 25 about the box centre, aim jittered by +-10 degrees."""
+    rng = np.random.default_rng(seed)
+    centre = 0.5 * (aabb[:3].astype(np.float64) + aabb[3:].astype(np.float64))
+    out = []
+    for _ in range(64):
+        w, h = RESOLUTIONS[rng.integers(0, 4)]
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        eye = centre + 25.0 * d
+        fwd = centre - eye
+        fwd /= np.linalg.norm(fwd)
+        yaw, pitch = np.radians(rng.uniform(-10, 10, 2))
+        up = np.array([0.0, 1.0, 0.0]) if abs(fwd[1]) < 0.99 else np.array([0.0, 0.0, 1.0])
+        right = np.cross(fwd, up)
+        right /= np.linalg.norm(right)
+        upv = np.cross(right, fwd)
+        fwd2 = fwd * math.cos(yaw) * math.cos(pitch) + right * math.sin(yaw) + upv * math.sin(pitch)
+        cam = K.look_at(eye, eye + fwd2, 60.0, w / h, 0.005)
+        out.append((K.camera_make(cam, w, h), w, h))
+    return out
+
+
+def algorithmic_bytes(counters):
+    """B_ray = 8*(node+leaf visits) + 52*tests + 256*evaluated hits + 16 (RGBA out),
+    reference data layout, summed over rays."""
+    c = counters.reshape(-1, K.N_COUNTERS).astype(np.int64)
+    return int(8 * (c[:, 0] + c[:, 1]).sum() + 52 * c[:, 2].sum() + 256 * c[:, 6].sum() + 16 * len(c))
+
+
+def load_traffic(workload_key):
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return json.load(f).get(workload_key)
+    except Exception:
+        return None
+
+
+def cpu_baseline(tree, g_inv, ci, w, h, budget_s):
+    import oracle_lib as O
+    threads = O.lib().oracle_max_threads()
+    nodes, offs, aabb = tree.nodes, tree.offsets, tree.aabb
+    times = []
+    t_start = time.time()
+    while len(times) < 5 and (time.time() - t_start) < budget_s:
+        t0 = time.time()
+        O.render(nodes, offs, g_inv, aabb, ci, w, h, threads=threads)
+        times.append(time.time() - t0)
+    med = float(np.median(times))
+    return {"value": round(w * h / med / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"full {w}x{h} view of the same 1M scene, FP32 oracle with OpenMP dynamic 16x16 tiles, "
+                      f"median of {len(times)} run(s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--gaussians", type=int, default=1_000_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); this build has no multi-GPU path.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    g, tree, build_s = scene_1m(args.gaussians)
+    st = tree.stats()
+    log(f"scene: {len(g)} Gaussians, {st['n_triangles']} proxy triangles, {st['n_nodes']} nodes, "
+        f"{st['n_offsets']} leaf refs, depth {st['max_depth']}, build {build_s:.1f} s")
+    aabb = tree.aabb
+    dev = K.Device(0)
+    info = dev.info()
+    sc = dev.scene(tree, g)
+    stream = dev.stream()
+    W, H = 1920, 1080
+    ci = view_1080p(aabb)
+    out = dev.alloc(W * H * 16)
+
+    for _ in range(args.warmup):
+        dev.render(sc, ci, W, H, out, stream=stream)
+    dev.sync()
+    # Timed region: K launches on `stream`, HIP events on that stream.
+    t0 = time.perf_counter()
+    dev.timer_begin(stream)
+    for _ in range(args.steps):
+        dev.render(sc, ci, W, H, out, stream=stream)
+    ms_total = dev.timer_end(stream)
+    dev.sync()
+    wall = time.perf_counter() - t0
+    ms_launch = ms_total / args.steps
+    rays = W * H
+    value = rays / (ms_launch * 1e-3) / 1e6
+    log(f"plain 1080p: {ms_launch:.3f} ms/launch, {value:.1f} Mrays/s (wall {wall:.3f} s for {args.steps})")
+
+    dbg = dev.render_debug(sc, ci, W, H)
+    cnt = dbg["counters"]
+    B = algorithmic_bytes(cnt)
+    achieved = B / (ms_launch * 1e-3) / 1e9
+    mean_c = cnt.reshape(-1, K.N_COUNTERS).mean(0)
+    log("mean counters/ray: " + ", ".join(f"{n}={v:.2f}" for n, v in zip(K.COUNTER_NAMES, mean_c)))
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": load_traffic("plain_1080p_1M"),
+            "algorithmic_bytes_per_launch": B, "bytes_per_ray": round(B / rays, 1), "kernel": "k_view<12,8,false,0>"}
+
+    result = {"metric": METRIC, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+              "warmup": args.warmup, "ms_per_step": round(ms_launch, 4), "higher_is_better": True, "scaling": "weak",
+              "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+              "config": {"workload": f"plain one-view launch: {len(g)} clustered Gaussians, one 1920x1080 pinhole "
+                                     f"view from radius 25, 1 ray/pixel, short stack 12",
+                         "gaussians": len(g), "width": W, "height": H, "stack_depth": 12},
+              "device": info["name"], "roofline": roof,
+              "build": {"host_sah_build_s": round(build_s, 2), "threads": os.cpu_count(), "nodes": st["n_nodes"],
+                        "leaf_refs": st["n_offsets"], "proxy_triangles": st["n_triangles"]}}
+
+    if not args.no_batched:
+        views = views_64(aabb)
+        bufs, vlist = [], []
+        total_rays = 0
+        for vci, w, h in views:
+            b = dev.alloc(w * h * 16)
+            bufs.append(b)
+            vlist.append(K.View(sc.h, vci, w, h, b.ptr))
+            total_rays += w * h
+        # plain: one launch per view, each waited for (the reference blocks per frame)
+        for (vci, w, h), b in zip(views, bufs):
+            dev.render(sc, vci, w, h, b, stream=stream)
+        dev.sync(stream)
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for (vci, w, h), b in zip(views, bufs):
+                dev.render(sc, vci, w, h, b, stream=stream)
+                dev.sync(stream)
+        plain_s = (time.perf_counter() - t0) / reps
+        # batched: autotune, then hipGraph replays of one persistent launch
+        tun, tms = K.autotune(dev, vlist, max_trials=20, reps=2)
+        batch = K.Batch(dev, vlist, (), tun)
+        batch.run(stream, use_graph=True)
+        dev.sync(stream)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            batch.run(stream, use_graph=True)
+            dev.sync(stream)
+        batch_s = (time.perf_counter() - t0) / reps
+        result["batched"] = {
+            "views": 64, "rays": total_rays,
+            "plain_per_view_mrays_s": round(total_rays / plain_s / 1e6, 2),
+            "batched_graph_mrays_s": round(total_rays / batch_s / 1e6, 2),
+            "speedup": round(plain_s / batch_s, 3), "tuning": tun.as_dict(), "autotune_best_ms": round(tms, 3)}
+        log(f"64 views ({total_rays} rays): plain {plain_s * 1e3:.2f} ms, batched {batch_s * 1e3:.2f} ms")
+        batch.free()
+        for b in bufs:
+            b.free()
+
+    if not args.no_cpu:
+        gi = K.invert_scales(g)
+        result["cpu_baseline"] = cpu_baseline(tree, gi, ci, W, H, args.cpu_budget)
+        log(f"cpu baseline: {result['cpu_baseline']}")
+
+    out.free()
+    sc.free()
+    dev.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()
