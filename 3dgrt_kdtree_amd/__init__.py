@@ -88,8 +88,8 @@ class RaySet(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch_views", C.c_int32),
-                ("max_inflight_batches", C.c_int32), ("streams", C.c_int32), ("tuning", Tuning),
-                ("batch_window_ms", C.c_double)]
+                ("max_inflight_batches", C.c_int32), ("plain_launches", C.c_int32), ("tuning", Tuning),
+                ("batch_window_ms", C.c_double), ("simulate_mrays_per_s", C.c_double)]
 
 
 class Request(C.Structure):

@@ -10,6 +10,7 @@
 //              from many views / ray sets / scenes; one queue per XCD with
 //              stealing, each wave pulls its own tile (no block barriers).
 //  k_reset     zeroes the queue counters (graph-safe, no memset node).
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,6 +22,7 @@
 #include <vector>
 
 #include "internal.hpp"
+#include "runtime.hpp"
 #include "trace_device.hpp"
 
 using namespace kdgrt_dev;
@@ -155,6 +157,11 @@ struct BatchArgs {
     int tile_w, tile_h;          // view tile in pixels; tile_w*tile_h is a multiple of 64
 };
 
+// Plain streaming copy, the calibration kernel for HBM traffic counters.
+__global__ __launch_bounds__(256) void k_copy(float4* __restrict__ dst, const float4* __restrict__ src, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) dst[i] = src[i];
+}
+
 __global__ void k_reset(uint32_t* c, int n) {
     if ((int)threadIdx.x < n) c[threadIdx.x] = 0;
 }
@@ -246,6 +253,7 @@ struct kdgrt_scene {
     void* d_offs = nullptr;
     void* d_geom = nullptr;
     void* d_shade = nullptr;
+    void* d_sph = nullptr;
     uint64_t bytes = 0;
     uint32_t n_nodes = 0;
     uint64_t n_offsets = 0, n_gaussians = 0;
@@ -283,7 +291,109 @@ kdgrt_status batch_build(kdgrt_context* ctx, const kdgrt_view* views, int n_view
                          int n_sets, const kdgrt_tuning* tuning, kdgrt_batch** out);
 kdgrt_status batch_launch(kdgrt_batch* b, hipStream_t s);
 
+// A reusable launch slot for the serving loop: device work table and queue
+// counters plus a pinned host staging table, all allocated once, so a launch
+// is an async table upload + reset + persistent kernel on the caller's
+// stream with no allocation or implicit device synchronisation.
+struct BatchSlot {
+    kdgrt_context* ctx = nullptr;
+    int max_items = 0;
+    WorkItem* h_items = nullptr; // pinned
+    WorkItem* d_items = nullptr;
+    uint32_t* d_counters = nullptr;
+};
+
 } // namespace kdgrt
+
+
+// ---------------------------------------------------------------------------
+// Bound-sphere records for the conservative pre-test (trace_device.hpp,
+// sphere_pass).  For a Gaussian with inverse scales g and rotation matrix M
+// (the device's float computation from the quaternion), normalised
+// coordinates are x' = A (x - mu) with A = diag(g) M.  A ray the exact test
+// accepts passes within sqrt(8) normalised units, i.e. within
+// sqrt(8) / sigma_min(A) world units of mu.  The t test uses world distances
+// only when M is orthogonal, so a non-orthogonal or singular M gets r = inf.
+// ---------------------------------------------------------------------------
+static void sym3_eigen(double a[3][3], double ev[3]) {
+    // cyclic Jacobi on a symmetric 3x3 matrix
+    for (int sweep = 0; sweep < 50; ++sweep) {
+        double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+        if (off < 1e-30 * (a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2]) || off == 0.0) break;
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) {
+                if (a[p][q] == 0.0) continue;
+                const double th = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < 3; ++k) {
+                    const double akp = a[k][p], akq = a[k][q];
+                    a[k][p] = c * akp - s * akq;
+                    a[k][q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    const double apk = a[p][k], aqk = a[q][k];
+                    a[p][k] = c * apk - s * aqk;
+                    a[q][k] = s * apk + c * aqk;
+                }
+            }
+    }
+    for (int k = 0; k < 3; ++k) ev[k] = a[k][k];
+}
+
+static uint16_t f32_to_f16_up(float x) {
+    // smallest half >= x for x >= 0 (inf and NaN map to +inf)
+    if (!(x <= 65504.0f)) return 0x7C00u;
+    if (x <= 0.0f) return 0;
+    __half h = __float2half_ru(x);
+    return __half_as_ushort(h);
+}
+
+static float4 bound_sphere(const float4& a, const float4& b, const float4& q) {
+    const float qr = q.x, qx = q.y, qy = q.z, qz = q.w;
+    const float xx = qx * qx, yy = qy * qy, zz = qz * qz;
+    const float xy = qx * qy, xz = qx * qz, yz = qy * qz;
+    const float rx = qr * qx, ry = qr * qy, rz = qr * qz;
+    const float mf[3][3] = {{(1.f - 2.f * (yy + zz)), 2.f * (xy + rz), 2.f * (xz - ry)},
+                            {2.f * (xy - rz), (1.f - 2.f * (xx + zz)), 2.f * (yz + rx)},
+                            {2.f * (xz + ry), 2.f * (yz - rx), (1.f - 2.f * (xx + yy))}};
+    const double g[3] = {b.x, b.y, b.z};
+    double r = INFINITY, kap = 0.0;
+    bool ok = std::isfinite(a.x) && std::isfinite(a.y) && std::isfinite(a.z);
+    double ortho = 0.0, A[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double mtm = 0.0;
+            for (int k = 0; k < 3; ++k) mtm += (double)mf[k][i] * (double)mf[k][j];
+            ortho = std::max(ortho, std::fabs(mtm - (i == j ? 1.0 : 0.0)));
+            A[i][j] = g[i] * (double)mf[i][j];
+            ok = ok && std::isfinite(A[i][j]);
+        }
+    if (ok && ortho <= 4e-6) {
+        double AtA[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double s2 = 0.0;
+                for (int k = 0; k < 3; ++k) s2 += A[k][i] * A[k][j];
+                AtA[i][j] = s2;
+            }
+        double ev[3];
+        sym3_eigen(AtA, ev);
+        const double lmin = std::min(ev[0], std::min(ev[1], ev[2]));
+        const double lmax = std::max(ev[0], std::max(ev[1], ev[2]));
+        if (lmin > 0.0 && std::isfinite(lmax)) {
+            const double smin = std::sqrt(lmin) * (1.0 - 1e-9), smax = std::sqrt(lmax);
+            r = std::sqrt(8.0) / smin * (1.0 + 1e-3);
+            kap = 8e-6 * (4.0 + smax / smin);
+        }
+    }
+    if (!std::isfinite(r)) kap = 0.0;
+    const uint32_t packed = (uint32_t)f32_to_f16_up((float)std::min(r, 1e30)) |
+                            ((uint32_t)f32_to_f16_up((float)std::min(kap, 1e30)) << 16);
+    float w;
+    std::memcpy(&w, &packed, 4);
+    return make_float4(a.x, a.y, a.z, w);
+}
 
 extern "C" {
 
@@ -407,9 +517,22 @@ kdgrt_status kdgrt_timer_end(kdgrt_context* ctx, void* stream, float* ms) {
     return KDGRT_OK;
 }
 
+kdgrt_status kdgrt_copy_f32x4(kdgrt_context* ctx, void* dst, const void* src, uint64_t n_float4, void* stream) {
+    if (!ctx || (n_float4 && (!dst || !src))) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "copy: null");
+    if (n_float4 == 0) return KDGRT_OK;
+    const int n_cu = ctx->prop.multiProcessorCount > 0 ? ctx->prop.multiProcessorCount : 256;
+    const uint64_t want = (n_float4 + 255) / 256;
+    const unsigned grid = (unsigned)std::min<uint64_t>(want, (uint64_t)n_cu * 16);
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, (hipStream_t)stream, (float4*)dst, (const float4*)src,
+                       n_float4);
+    HIP_TRY(hipGetLastError());
+    return KDGRT_OK;
+}
+
 uint64_t kdgrt_scene_bytes_estimate(uint32_t n_nodes, uint64_t n_offsets, uint64_t n_gaussians) {
     auto r = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
-    return r(8ull * n_nodes) + r(4ull * n_offsets) + r(48ull * n_gaussians) + r(192ull * n_gaussians);
+    return r(8ull * n_nodes) + r(4ull * n_offsets) + r(48ull * n_gaussians) + r(192ull * n_gaussians) +
+           r(16ull * n_gaussians);
 }
 
 kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uint32_t n_nodes,
@@ -423,7 +546,7 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
     HIP_TRY(hipSetDevice(ctx->device));
     // Repack: 48-B geometry records (pos|opacity, inverse scale|k_scale,
     // quaternion) and 192-B SH records.
-    std::vector<float4> geom(3 * n_g), shade(12 * n_g);
+    std::vector<float4> geom(3 * n_g), shade(12 * n_g), sph(n_g);
     for (uint64_t i = 0; i < n_g; ++i) {
         const kdgrt_gaussian& s = g[i];
         float inv[3], kinv;
@@ -442,6 +565,7 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
         geom[3 * i + 1] = make_float4(inv[0], inv[1], inv[2], kinv);
         geom[3 * i + 2] = make_float4(s.rot[0], s.rot[1], s.rot[2], s.rot[3]);
         std::memcpy(&shade[12 * i], s.f_dc, 48 * sizeof(float));
+        sph[i] = bound_sphere(geom[3 * i + 0], geom[3 * i + 1], geom[3 * i + 2]);
     }
     auto* sc = new kdgrt_scene();
     sc->ctx = ctx;
@@ -450,6 +574,7 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
         if (sc->d_offs) hipFree(sc->d_offs);
         if (sc->d_geom) hipFree(sc->d_geom);
         if (sc->d_shade) hipFree(sc->d_shade);
+        if (sc->d_sph) hipFree(sc->d_sph);
         delete sc;
         return set_error(e == hipErrorOutOfMemory ? KDGRT_ERR_OUT_OF_MEMORY : KDGRT_ERR_DEVICE,
                          std::string("scene_create: ") + what + ": " + hipGetErrorString(e));
@@ -459,6 +584,9 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
     if ((e = hipMalloc(&sc->d_offs, std::max<uint64_t>(4, 4ull * n_offsets))) != hipSuccess) return fail(e, "offsets");
     if ((e = hipMalloc(&sc->d_geom, 48ull * n_g)) != hipSuccess) return fail(e, "geometry");
     if ((e = hipMalloc(&sc->d_shade, 192ull * n_g)) != hipSuccess) return fail(e, "sh");
+    if ((e = hipMalloc(&sc->d_sph, 16ull * n_g)) != hipSuccess) return fail(e, "spheres");
+    if ((e = hipMemcpy(sc->d_sph, sph.data(), 16ull * n_g, hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(e, "copy");
     if ((e = hipMemcpy(sc->d_nodes, nodes, 8ull * n_nodes, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "copy");
     if (n_offsets && (e = hipMemcpy(sc->d_offs, offsets, 4ull * n_offsets, hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "copy");
@@ -469,6 +597,7 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
     sc->dev.offs = (const uint32_t*)sc->d_offs;
     sc->dev.geom = (const float4*)sc->d_geom;
     sc->dev.shade = (const float4*)sc->d_shade;
+    sc->dev.sph = (const float4*)sc->d_sph;
     for (int k = 0; k < 3; ++k) {
         sc->dev.bmin[k] = aabb[k];
         sc->dev.bmax[k] = aabb[3 + k];
@@ -495,6 +624,7 @@ kdgrt_status kdgrt_scene_destroy(kdgrt_scene* sc) {
     hipFree(sc->d_offs);
     hipFree(sc->d_geom);
     hipFree(sc->d_shade);
+    hipFree(sc->d_sph);
     delete sc;
     return KDGRT_OK;
 }
@@ -806,6 +936,80 @@ kdgrt_status batch_launch(kdgrt_batch* b, hipStream_t s) {
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(b->kernel, dim3(b->grid), dim3(b->block), 0, s, b->args);
     HIP_TRY(hipGetLastError());
+    return KDGRT_OK;
+}
+
+} // namespace kdgrt
+
+namespace kdgrt {
+
+int context_device(const kdgrt_context* ctx) { return ctx ? ctx->device : 0; }
+
+kdgrt_status slot_create(kdgrt_context* ctx, int max_items, BatchSlot** out) {
+    if (!ctx || !out || max_items < 1) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "slot_create: bad argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto* sl = new BatchSlot();
+    sl->ctx = ctx;
+    sl->max_items = max_items;
+    hipError_t e = hipHostMalloc((void**)&sl->h_items, sizeof(WorkItem) * max_items, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&sl->d_items, sizeof(WorkItem) * max_items);
+    if (e == hipSuccess) e = hipMalloc(&sl->d_counters, 64 * sizeof(uint32_t));
+    if (e != hipSuccess) {
+        slot_destroy(sl);
+        return set_error(KDGRT_ERR_OUT_OF_MEMORY, std::string("slot_create: ") + hipGetErrorString(e));
+    }
+    *out = sl;
+    return KDGRT_OK;
+}
+
+void slot_destroy(BatchSlot* sl) {
+    if (!sl) return;
+    if (sl->h_items) hipHostFree(sl->h_items);
+    if (sl->d_items) hipFree(sl->d_items);
+    if (sl->d_counters) hipFree(sl->d_counters);
+    delete sl;
+}
+
+kdgrt_status slot_launch(BatchSlot* sl, const kdgrt_view* views, int n, const kdgrt_tuning& t, hipStream_t s,
+                         uint64_t* rays) {
+    if (!sl || n < 1 || n > sl->max_items) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "slot_launch: bad batch");
+    if (t.tile_w < 8 || t.tile_h < 8 || t.tile_w % 8 || t.tile_h % 8 || t.tile_w * t.tile_h > 4096 ||
+        t.stack_depth < 1 || t.stack_depth > 32 || (t.hit_capacity != 4 && t.hit_capacity != 8 && t.hit_capacity != 16) ||
+        (t.block_threads != 64 && t.block_threads != 128 && t.block_threads != 256) || t.waves_per_cu < 1)
+        return set_error(KDGRT_ERR_INVALID_ARGUMENT, "slot_launch: bad tuning");
+    uint64_t tiles = 0, nr = 0;
+    for (int i = 0; i < n; ++i) {
+        const kdgrt_view& v = views[i];
+        WorkItem w{};
+        w.sc = v.scene->dev;
+        std::memcpy(&w.cam, &v.cam, sizeof(CamInfo));
+        w.W = v.width;
+        w.H = v.height;
+        w.tiles_x = (v.width + t.tile_w - 1) / t.tile_w;
+        w.tile_begin = (uint32_t)tiles;
+        w.tile_count = (uint32_t)w.tiles_x * (uint32_t)((v.height + t.tile_h - 1) / t.tile_h);
+        w.out = (float4*)v.out_rgbt;
+        tiles += w.tile_count;
+        nr += (uint64_t)v.width * v.height;
+        sl->h_items[i] = w;
+    }
+    HIP_TRY(hipMemcpyAsync(sl->d_items, sl->h_items, sizeof(WorkItem) * n, hipMemcpyHostToDevice, s));
+    BatchArgs a{};
+    a.items = sl->d_items;
+    a.n_items = n;
+    for (int x = 0; x <= kXcds; ++x) a.q_begin[x] = (uint32_t)((uint64_t)tiles * x / kXcds);
+    a.counters = sl->d_counters;
+    a.depth = t.stack_depth;
+    a.tile_w = t.tile_w;
+    a.tile_h = t.tile_h;
+    const int n_cu = sl->ctx->prop.multiProcessorCount > 0 ? sl->ctx->prop.multiProcessorCount : 256;
+    int grid = std::max(1, t.waves_per_cu * n_cu * 64 / t.block_threads);
+    grid = std::min<int>(grid, (int)std::max<uint64_t>(1, (tiles * 64 + t.block_threads - 1) / t.block_threads));
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, sl->d_counters, kXcds);
+    hipLaunchKernelGGL(pick_batch_kernel(t.stack_depth <= 12 ? 12 : 32, t.hit_capacity), dim3(grid),
+                       dim3(t.block_threads), 0, s, a);
+    HIP_TRY(hipGetLastError());
+    if (rays) *rays = nr;
     return KDGRT_OK;
 }
 

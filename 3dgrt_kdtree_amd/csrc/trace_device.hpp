@@ -32,6 +32,7 @@
 //    computed once per ray with the reference's double-precision products.
 #pragma once
 
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -44,6 +45,7 @@ struct DevScene {
     const uint32_t* __restrict__ offs;
     const float4* __restrict__ geom;  // 3 x float4 per Gaussian: pos|opacity, inv scale|k_scale, quat wxyz
     const float4* __restrict__ shade; // 12 x float4 per Gaussian: f_dc[3], f_rest[45]
+    const float4* __restrict__ sph;   // 1 x float4 per Gaussian: centre, packed (radius, kappa) as 2 x f16
     float bmin[3];
     float bmax[3];
 };
@@ -135,6 +137,26 @@ __device__ __forceinline__ void sh_radiance(const float4* __restrict__ sh, const
         r = r + 0.5f;
         rad[ch] = fmaxf(r, 0.0f);
     }
+}
+
+// Conservative pre-test.  Any Gaussian that the exact test below can accept
+// lies within radius r of the ray in world space (r bounds sqrt(8) normalised
+// units mapped back through diag(1/scale)*M, with a relative margin), and its
+// t lies within r of the projection of the centre; kappa*|a|_1 covers the
+// float rounding of both the exact test and this one.  r = inf disables the
+// pre-test for a Gaussian (singular or non-orthogonal rotation).  The exact
+// test alone decides acceptance, so the pre-test changes no result.
+__device__ __forceinline__ bool sphere_pass(const float o[3], const float d[3], const float4 sp, float t_near,
+                                            float t_far) {
+    const float ax = sp.x - o[0], ay = sp.y - o[1], az = sp.z - o[2];
+    const uint32_t w = __float_as_uint(sp.w);
+    const float r = __half2float(__ushort_as_half((unsigned short)(w & 0xFFFFu)));
+    const float kap = __half2float(__ushort_as_half((unsigned short)(w >> 16)));
+    const float R = r + kap * (fabsf(ax) + fabsf(ay) + fabsf(az));
+    const float tp = ax * d[0] + ay * d[1] + az * d[2];
+    const float cx = ay * d[2] - az * d[1], cy = az * d[0] - ax * d[2], cz = ax * d[1] - ay * d[0];
+    const float dist2 = cx * cx + cy * cy + cz * cz;
+    return (dist2 <= R * R) & (tp + R >= t_near) & (tp - R <= t_far);
 }
 
 struct TestOut {
@@ -272,41 +294,63 @@ __device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], 
             }
             int n = 0;
             bool overflow = false;
-            for (uint32_t s = 0; s < count; ++s) {
-                const uint32_t gid = sc.offs[base + s];
-                const TestOut r = ray_gaussian(o, d, sc.geom + 3 * (size_t)gid, t_near, t_far);
-                if (!r.hit) continue;
-                if (have_lb && !(r.t > lb_t || (r.t == lb_t && s > lb_s))) continue;
-                if (DBG && !have_lb) first_hits++;
-                int pos = 0;
+            // Windows of 64 slots: the cheap pre-test runs for every slot and
+            // builds a survivor mask; the exact test then runs only for
+            // survivors, in slot order, so a wavefront executes the exact test
+            // as often as its busiest lane needs it rather than once for every
+            // slot some lane still needs.
+            for (uint32_t w0 = 0; w0 < count; w0 += 64) {
+                const uint32_t wn = (count - w0) < 64u ? (count - w0) : 64u;
+                uint64_t mask = 0;
+                for (uint32_t j0 = 0; j0 < wn; j0 += 4) {
+                    uint32_t gids[4];
 #pragma unroll
-                for (int i = 0; i < K; ++i) pos += (i < n && kt[i] <= r.t) ? 1 : 0;
-                if (pos >= K) {
-                    overflow = true;
-                    continue;
-                }
-                if (n == K) overflow = true;
+                    for (int j = 0; j < 4; ++j) gids[j] = (j0 + j < wn) ? sc.offs[base + w0 + j0 + j] : 0u;
 #pragma unroll
-                for (int i = K - 1; i > 0; --i) {
-                    if (i > pos) {
-                        kt[i] = kt[i - 1];
-                        kg[i] = kg[i - 1];
-                        ko[i] = ko[i - 1];
-                        ks[i] = ks[i - 1];
-                    } else if (i == pos) {
-                        kt[i] = r.t;
-                        kg[i] = r.gd;
-                        ko[i] = r.op;
-                        ks[i] = s;
+                    for (int j = 0; j < 4; ++j) {
+                        if (j0 + j < wn && sphere_pass(o, d, sc.sph[gids[j]], t_near, t_far))
+                            mask |= 1ull << (j0 + j);
                     }
                 }
-                if (pos == 0) {
-                    kt[0] = r.t;
-                    kg[0] = r.gd;
-                    ko[0] = r.op;
-                    ks[0] = s;
+                while (mask) {
+                    const uint32_t j = (uint32_t)__builtin_ctzll(mask);
+                    mask &= mask - 1;
+                    const uint32_t s = w0 + j;
+                    const uint32_t gid = sc.offs[base + s];
+                    const TestOut r = ray_gaussian(o, d, sc.geom + 3 * (size_t)gid, t_near, t_far);
+                    if (!r.hit) continue;
+                    if (have_lb && !(r.t > lb_t || (r.t == lb_t && s > lb_s))) continue;
+                    if (DBG && !have_lb) first_hits++;
+                    int pos = 0;
+#pragma unroll
+                    for (int i = 0; i < K; ++i) pos += (i < n && kt[i] <= r.t) ? 1 : 0;
+                    if (pos >= K) {
+                        overflow = true;
+                        continue;
+                    }
+                    if (n == K) overflow = true;
+#pragma unroll
+                    for (int i = K - 1; i > 0; --i) {
+                        if (i > pos) {
+                            kt[i] = kt[i - 1];
+                            kg[i] = kg[i - 1];
+                            ko[i] = ko[i - 1];
+                            ks[i] = ks[i - 1];
+                        } else if (i == pos) {
+                            kt[i] = r.t;
+                            kg[i] = r.gd;
+                            ko[i] = r.op;
+                            ks[i] = s;
+                        }
+                    }
+                    if (pos == 0) {
+                        kt[0] = r.t;
+                        kg[0] = r.gd;
+                        ko[0] = r.op;
+                        ks[0] = s;
+                    }
+                    n = n + 1 < K ? n + 1 : K;
                 }
-                n = n + 1 < K ? n + 1 : K;
             }
             if (DBG && !have_lb) {
                 dbg->cnt[3] += first_hits;

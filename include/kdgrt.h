@@ -169,6 +169,12 @@ KDGRT_API kdgrt_status kdgrt_timer_end(kdgrt_context* ctx, void* stream, float* 
 KDGRT_API kdgrt_status kdgrt_device_info(kdgrt_context* ctx, char* name, size_t name_len,
                                          int* n_cu, size_t* total_mem, size_t* lds_per_block);
 
+/* Streaming device copy of n_float4 x 16 B (grid-stride, 256-thread blocks).
+ * No reference counterpart: the calibration kernel whose HBM counter
+ * readings (FETCH_SIZE, WRITE_SIZE) scale the render kernel's. */
+KDGRT_API kdgrt_status kdgrt_copy_f32x4(kdgrt_context* ctx, void* dst, const void* src, uint64_t n_float4,
+                                        void* stream);
+
 /* ---------------- scenes (device-resident, immutable) ---------------- */
 /* nodes/offsets/aabb as in a .kdt file; aabb = xmin,ymin,zmin,xmax,ymax,zmax.
  * scales_inverted = 1 when `g` already holds 1/scale (the reference caller
@@ -265,12 +271,13 @@ KDGRT_API kdgrt_status kdgrt_autotune(kdgrt_context* ctx, const kdgrt_view* view
 
 /* ---------------- serving loop ---------------- */
 typedef struct kdgrt_server_config {
-    uint64_t hbm_budget_bytes;   /* scenes + in-flight request buffers */
+    uint64_t hbm_budget_bytes;   /* resident scenes + the output arena of admitted requests */
     int32_t max_batch_views;     /* views per persistent launch */
-    int32_t max_inflight_batches;/* launches overlapping on the device */
-    int32_t streams;             /* HIP streams used for overlap */
+    int32_t max_inflight_batches;/* launches overlapping on the device (one stream each) */
+    int32_t plain_launches;      /* 1 = comparison mode: one plain launch per request, one at a time */
     kdgrt_tuning tuning;
-    double batch_window_ms;      /* wait this long for more arrivals once one is queued */
+    double batch_window_ms;      /* while the device is busy, wait up to this long to fill a batch */
+    double simulate_mrays_per_s; /* > 0 with ctx == NULL: no device, batches take rays / rate (tests) */
 } kdgrt_server_config;
 
 typedef struct kdgrt_request {
@@ -299,15 +306,19 @@ typedef struct kdgrt_server_stats {
 
 KDGRT_API kdgrt_status kdgrt_server_create(kdgrt_context* ctx, const kdgrt_server_config* cfg,
                                            kdgrt_server** server);
-/* Makes a scene resident (counted against the budget). */
+/* Makes a scene resident (counted against the budget); before start only. */
 KDGRT_API kdgrt_status kdgrt_server_add_scene(kdgrt_server* s, const kdgrt_tree* tree, const kdgrt_gaussian* g,
                                               uint64_t n_gaussians, int scales_inverted, int32_t* scene_id);
 KDGRT_API kdgrt_status kdgrt_server_remove_scene(kdgrt_server* s, int32_t scene_id);
 KDGRT_API kdgrt_status kdgrt_server_start(kdgrt_server* s);
-/* Thread-safe.  Returns KDGRT_ERR_CAPACITY when the request can never fit the budget. */
+/* Thread-safe.  Returns KDGRT_ERR_CAPACITY when the request can never fit the
+ * output arena (budget minus resident scenes). */
 KDGRT_API kdgrt_status kdgrt_server_submit(kdgrt_server* s, const kdgrt_request* req, uint64_t* request_id);
-/* Waits up to timeout_ms for request completion; copies the RGBT image
- * (width*height float4) to host memory `host_rgbt` if not NULL. */
+/* Waits up to timeout_ms (< 0: forever) for the request; on completion copies
+ * the RGBT image (width*height float4, reference row flip) to `host_rgbt` if
+ * not NULL and releases the request's device memory.  Every completed
+ * request must be waited for once: its output occupies the arena until then.
+ * Returns KDGRT_ERR_TIMEOUT if not finished in time (the request stays). */
 KDGRT_API kdgrt_status kdgrt_server_wait(kdgrt_server* s, uint64_t request_id, double timeout_ms,
                                          float* host_rgbt, kdgrt_result_info* info);
 KDGRT_API kdgrt_status kdgrt_server_cancel(kdgrt_server* s, uint64_t request_id);
