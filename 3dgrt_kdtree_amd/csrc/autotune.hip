@@ -62,7 +62,7 @@ extern "C" kdgrt_status kdgrt_autotune(kdgrt_context* ctx, const kdgrt_view* vie
         std::vector<int> vals;
     };
     const std::vector<Dim> dims = {
-        {0, {4, 8, 16, 32}},      // waves per CU
+        {0, {8, 16, 24, 32, 48}}, // waves per CU
         {1, {0, 1, 2}},           // tile: 8x8, 16x8, 16x16
         {2, {4, 8, 16}},          // hit capacity
         {3, {12, 8, 16}},         // stack depth

@@ -2,10 +2,10 @@
 //
 // Kernels:
 //  k_view      plain one-view launch (cudaRenderer.cu:2313-2390 + 3099-3484):
-//              256-thread blocks = four 8x8-pixel wave tiles; block index
-//              remapped so each XCD walks a contiguous band of the image
-//              (one L2 per XCD keeps that band's nodes and Gaussians).
-//  k_rays      explicit ray lists, 64 rays per wave.
+//              one-wave blocks of 8x8 pixels, 64x64-pixel groups of tiles
+//              interleaved over the XCDs (each L2 keeps a compact
+//              neighbourhood, the costly image centre is spread over all).
+//  k_rays      explicit ray lists, 64 rays per one-wave block.
 //  k_batch     persistent launch over a device work queue of ray tiles drawn
 //              from many views / ray sets / scenes; one queue per XCD with
 //              stealing, each wave pulls its own tile (no block barriers).
@@ -35,6 +35,7 @@ constexpr int kXcds = 8;
 // Plain one-view kernel.
 // ---------------------------------------------------------------------------
 struct ViewArgs {
+    uint64_t* prof; // 5 x u64 per ray (profile variant)
     DevScene sc;
     CamInfo cam;
     int W, H;
@@ -47,23 +48,35 @@ struct ViewArgs {
     int seq_cap;
 };
 
-__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
-    // Hardware dispatches workgroup b to XCD b % 8.  Give XCD x the
-    // contiguous logical range [x*q + min(x,r), ...) so neighbouring tiles
-    // share an L2.
-    const uint32_t q = nb / kXcds, r = nb % kXcds;
-    const uint32_t x = b % kXcds, i = b / kXcds;
-    return x * q + (x < r ? x : r) + i;
+// Hardware dispatches workgroup b to XCD b % 8.  Logical blocks are taken
+// in groups of G that share one XCD (and its L2), and consecutive groups go
+// round-robin over the XCDs, so each L2 sees compact neighbourhoods while the
+// expensive middle of an image is spread over all XCDs.  The grid is rounded
+// up to a multiple of 8G; logical indices past the end exit at once.
+template <uint32_t G>
+__device__ __forceinline__ uint32_t xcd_interleave(uint32_t b) {
+    const uint32_t i = b / kXcds;
+    return (i / G) * (G * kXcds) + (b % kXcds) * G + (i % G);
 }
 
-template <int DMAX, int K, bool DBG, int FMT>
-__global__ __launch_bounds__(256) void k_view(ViewArgs a) {
-    const uint32_t nb = (uint32_t)a.blocks_x * (uint32_t)a.blocks_y;
-    const uint32_t lb = xcd_remap(blockIdx.x, nb);
-    const int bx = (int)(lb % (uint32_t)a.blocks_x), by = (int)(lb / (uint32_t)a.blocks_x);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int x = bx * 16 + (wave & 1) * 8 + (lane & 7);
-    const int y = by * 16 + (wave >> 1) * 8 + (lane >> 3);
+template <uint32_t G>
+static inline uint32_t interleave_grid(uint64_t n_logical) {
+    return (uint32_t)(((n_logical + G * kXcds - 1) / (G * kXcds)) * (G * kXcds));
+}
+
+template <int K, bool DBG, int FMT, bool PROF = false>
+__global__ __launch_bounds__(64) void k_view(ViewArgs a) {
+    // One wavefront per block = one 8x8-pixel tile, so a finished wave frees
+    // its slot at once (per-ray cost varies widely across an image).  Groups
+    // of 64 tiles (64x64 px) share an XCD.
+    const uint32_t L = xcd_interleave<64>(blockIdx.x);
+    const uint32_t st = L / 64u, local = L % 64u;
+    if (st >= (uint32_t)a.blocks_x * (uint32_t)a.blocks_y) return;
+    const int tx = (int)(st % (uint32_t)a.blocks_x) * 8 + (int)(local % 8u);
+    const int ty = (int)(st / (uint32_t)a.blocks_x) * 8 + (int)(local / 8u);
+    const int lane = threadIdx.x;
+    const int x = tx * 8 + (lane & 7);
+    const int y = ty * 8 + (lane >> 3);
     if (x >= a.W || y >= a.H) return;
     float o[3], d[3], rgb[3], A;
     make_ray(a.cam, x, y, o, d);
@@ -76,7 +89,14 @@ __global__ __launch_bounds__(256) void k_view(ViewArgs a) {
         dbg.seq_cap = a.seq_cap;
         dbg.nseq = 0;
     }
-    trace_ray<DMAX, K, DBG>(a.sc, o, d, a.depth, rgb, A, &dbg);
+    if (PROF) {
+        for (int i = 0; i < 5; ++i) dbg.cyc[i] = 0;
+        dbg.tlast = clock64();
+    }
+    extern __shared__ uint2 lds_stack[];
+    trace_ray<K, DBG, PROF>(a.sc, o, d, a.depth, lds_stack + lane, rgb, A, &dbg);
+    if (PROF)
+        for (int i = 0; i < 5; ++i) a.prof[5 * rid + i] = dbg.cyc[i];
     const size_t pid = (size_t)(a.H - y - 1) * a.W + x; // framebuffer row flip (cudaRenderer.cu:2367)
     if (FMT == KDGRT_OUT_RGBT_F32) {
         reinterpret_cast<float4*>(a.out)[pid] = make_float4(rgb[0], rgb[1], rgb[2], 1.0f - A);
@@ -107,10 +127,9 @@ struct RaysArgs {
     uint64_t* hashes;
 };
 
-template <int DMAX, int K, bool DBG>
-__global__ __launch_bounds__(256) void k_rays(RaysArgs a) {
-    const uint32_t nb = gridDim.x;
-    const uint64_t i = (uint64_t)xcd_remap(blockIdx.x, nb) * 256 + threadIdx.x;
+template <int K, bool DBG>
+__global__ __launch_bounds__(64) void k_rays(RaysArgs a) {
+    const uint64_t i = (uint64_t)xcd_interleave<64>(blockIdx.x) * 64 + threadIdx.x;
     if (i >= a.n) return;
     const float o[3] = {a.ro[3 * i], a.ro[3 * i + 1], a.ro[3 * i + 2]};
     const float d[3] = {a.rd[3 * i], a.rd[3 * i + 1], a.rd[3 * i + 2]};
@@ -123,7 +142,8 @@ __global__ __launch_bounds__(256) void k_rays(RaysArgs a) {
         dbg.seq_cap = 0;
         dbg.nseq = 0;
     }
-    trace_ray<DMAX, K, DBG>(a.sc, o, d, a.depth, rgb, A, &dbg);
+    extern __shared__ uint2 lds_stack[];
+    trace_ray<K, DBG>(a.sc, o, d, a.depth, lds_stack + threadIdx.x, rgb, A, &dbg);
     a.out[i] = make_float4(rgb[0], rgb[1], rgb[2], 1.0f - A);
     if (DBG) {
         if (a.counters)
@@ -166,9 +186,11 @@ __global__ void k_reset(uint32_t* c, int n) {
     if ((int)threadIdx.x < n) c[threadIdx.x] = 0;
 }
 
-template <int DMAX, int K>
+template <int K>
 __global__ __launch_bounds__(256) void k_batch(BatchArgs a) {
     const int lane = threadIdx.x & 63;
+    extern __shared__ uint2 lds_stack[];
+    uint2* const stk = lds_stack + (threadIdx.x >> 6) * (64 * a.depth) + lane;
     const int home = blockIdx.x % kXcds;
     int probe = 0;
     while (probe < kXcds) {
@@ -176,7 +198,9 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs a) {
         const uint32_t qlen = a.q_begin[q + 1] - a.q_begin[q];
         uint32_t t = 0;
         if (lane == 0) t = atomicAdd(&a.counters[q], 1u);
-        t = __shfl(t, 0);
+        // readfirstlane makes the tile (and everything derived from it)
+        // provably wave-uniform, so the work item lives in scalar registers
+        t = __builtin_amdgcn_readfirstlane(t);
         if (t >= qlen) {
             ++probe; // this queue is drained; move on for good
             continue;
@@ -189,7 +213,7 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs a) {
             if (a.items[mid].tile_begin <= tile) lo = mid;
             else hi = mid - 1;
         }
-        const WorkItem& it = a.items[lo];
+        const WorkItem it = a.items[__builtin_amdgcn_readfirstlane(lo)];
         const uint32_t local = tile - it.tile_begin;
         if (it.W > 0) {
             const int tx = (int)(local % (uint32_t)it.tiles_x), ty = (int)(local / (uint32_t)it.tiles_x);
@@ -203,7 +227,7 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs a) {
                 if (x >= it.W || y >= it.H) continue;
                 float o[3], d[3], rgb[3], A;
                 make_ray(it.cam, x, y, o, d);
-                trace_ray<DMAX, K, false>(it.sc, o, d, a.depth, rgb, A, nullptr);
+                trace_ray<K, false>(it.sc, o, d, a.depth, stk, rgb, A, nullptr);
                 it.out[(size_t)(it.H - y - 1) * it.W + x] = make_float4(rgb[0], rgb[1], rgb[2], 1.0f - A);
             }
         } else {
@@ -214,7 +238,7 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs a) {
                 const float o[3] = {it.ro[3 * i], it.ro[3 * i + 1], it.ro[3 * i + 2]};
                 const float d[3] = {it.rd[3 * i], it.rd[3 * i + 1], it.rd[3 * i + 2]};
                 float rgb[3], A;
-                trace_ray<DMAX, K, false>(it.sc, o, d, a.depth, rgb, A, nullptr);
+                trace_ray<K, false>(it.sc, o, d, a.depth, stk, rgb, A, nullptr);
                 it.out[i] = make_float4(rgb[0], rgb[1], rgb[2], 1.0f - A);
             }
         }
@@ -223,15 +247,15 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs a) {
 
 typedef void (*BatchKernel)(BatchArgs);
 
-BatchKernel pick_batch_kernel(int dmax, int k) {
-    if (dmax <= 12) {
-        if (k == 4) return k_batch<12, 4>;
-        if (k == 16) return k_batch<12, 16>;
-        return k_batch<12, 8>;
-    }
-    if (k == 4) return k_batch<32, 4>;
-    if (k == 16) return k_batch<32, 16>;
-    return k_batch<32, 8>;
+// Dynamic LDS of a launch: one 8-B short-stack ring entry per lane and level.
+static inline size_t stack_lds_bytes(int depth, int block_threads) {
+    return (size_t)depth * 8u * (size_t)block_threads;
+}
+
+BatchKernel pick_batch_kernel(int k) {
+    if (k == 4) return k_batch<4>;
+    if (k == 16) return k_batch<16>;
+    return k_batch<8>;
 }
 
 } // namespace
@@ -254,6 +278,7 @@ struct kdgrt_scene {
     void* d_geom = nullptr;
     void* d_shade = nullptr;
     void* d_sph = nullptr;
+    void* d_lsph = nullptr;
     uint64_t bytes = 0;
     uint32_t n_nodes = 0;
     uint64_t n_offsets = 0, n_gaussians = 0;
@@ -532,7 +557,7 @@ kdgrt_status kdgrt_copy_f32x4(kdgrt_context* ctx, void* dst, const void* src, ui
 uint64_t kdgrt_scene_bytes_estimate(uint32_t n_nodes, uint64_t n_offsets, uint64_t n_gaussians) {
     auto r = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
     return r(8ull * n_nodes) + r(4ull * n_offsets) + r(48ull * n_gaussians) + r(192ull * n_gaussians) +
-           r(16ull * n_gaussians);
+           r(16ull * n_gaussians) + r(16ull * n_offsets);
 }
 
 kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uint32_t n_nodes,
@@ -546,7 +571,7 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
     HIP_TRY(hipSetDevice(ctx->device));
     // Repack: 48-B geometry records (pos|opacity, inverse scale|k_scale,
     // quaternion) and 192-B SH records.
-    std::vector<float4> geom(3 * n_g), shade(12 * n_g), sph(n_g);
+    std::vector<float4> geom(3 * n_g), shade(12 * n_g), sph(n_g), lsph(std::max<uint64_t>(1, n_offsets));
     for (uint64_t i = 0; i < n_g; ++i) {
         const kdgrt_gaussian& s = g[i];
         float inv[3], kinv;
@@ -564,9 +589,14 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
         geom[3 * i + 0] = make_float4(s.pos[0], s.pos[1], s.pos[2], s.opacity);
         geom[3 * i + 1] = make_float4(inv[0], inv[1], inv[2], kinv);
         geom[3 * i + 2] = make_float4(s.rot[0], s.rot[1], s.rot[2], s.rot[3]);
-        std::memcpy(&shade[12 * i], s.f_dc, 48 * sizeof(float));
+        float* sh = reinterpret_cast<float*>(&shade[12 * i]); // channel-major SH record
+        for (int ch = 0; ch < 3; ++ch) {
+            sh[16 * ch] = s.f_dc[ch];
+            for (int k = 0; k < 15; ++k) sh[16 * ch + 1 + k] = s.f_rest[15 * ch + k];
+        }
         sph[i] = bound_sphere(geom[3 * i + 0], geom[3 * i + 1], geom[3 * i + 2]);
     }
+    for (uint64_t k = 0; k < n_offsets; ++k) lsph[k] = sph[offsets[k]]; // leaf-slot order
     auto* sc = new kdgrt_scene();
     sc->ctx = ctx;
     auto fail = [&](hipError_t e, const char* what) {
@@ -575,6 +605,7 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
         if (sc->d_geom) hipFree(sc->d_geom);
         if (sc->d_shade) hipFree(sc->d_shade);
         if (sc->d_sph) hipFree(sc->d_sph);
+        if (sc->d_lsph) hipFree(sc->d_lsph);
         delete sc;
         return set_error(e == hipErrorOutOfMemory ? KDGRT_ERR_OUT_OF_MEMORY : KDGRT_ERR_DEVICE,
                          std::string("scene_create: ") + what + ": " + hipGetErrorString(e));
@@ -585,6 +616,9 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
     if ((e = hipMalloc(&sc->d_geom, 48ull * n_g)) != hipSuccess) return fail(e, "geometry");
     if ((e = hipMalloc(&sc->d_shade, 192ull * n_g)) != hipSuccess) return fail(e, "sh");
     if ((e = hipMalloc(&sc->d_sph, 16ull * n_g)) != hipSuccess) return fail(e, "spheres");
+    if ((e = hipMalloc(&sc->d_lsph, 16ull * lsph.size())) != hipSuccess) return fail(e, "leaf spheres");
+    if ((e = hipMemcpy(sc->d_lsph, lsph.data(), 16ull * lsph.size(), hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(e, "copy");
     if ((e = hipMemcpy(sc->d_sph, sph.data(), 16ull * n_g, hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "copy");
     if ((e = hipMemcpy(sc->d_nodes, nodes, 8ull * n_nodes, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "copy");
@@ -598,6 +632,7 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
     sc->dev.geom = (const float4*)sc->d_geom;
     sc->dev.shade = (const float4*)sc->d_shade;
     sc->dev.sph = (const float4*)sc->d_sph;
+    sc->dev.lsph = (const float4*)sc->d_lsph;
     for (int k = 0; k < 3; ++k) {
         sc->dev.bmin[k] = aabb[k];
         sc->dev.bmax[k] = aabb[3 + k];
@@ -625,6 +660,7 @@ kdgrt_status kdgrt_scene_destroy(kdgrt_scene* sc) {
     hipFree(sc->d_geom);
     hipFree(sc->d_shade);
     hipFree(sc->d_sph);
+    hipFree(sc->d_lsph);
     delete sc;
     return KDGRT_OK;
 }
@@ -646,13 +682,14 @@ static kdgrt_status check_view(const kdgrt_context* ctx, const kdgrt_scene* sc, 
 
 static void fill_view_args(ViewArgs& a, const kdgrt_scene* sc, const kdgrt_camera_info* cam, int W, int H,
                            int depth, void* out) {
+    a.prof = nullptr;
     a.sc = sc->dev;
     std::memcpy(&a.cam, cam, sizeof(CamInfo));
     a.W = W;
     a.H = H;
     a.depth = depth;
-    a.blocks_x = (W + 15) / 16;
-    a.blocks_y = (H + 15) / 16;
+    a.blocks_x = (W + 63) / 64; // 64x64-pixel groups of 64 8x8 tiles
+    a.blocks_y = (H + 63) / 64;
     a.out = out;
     a.counters = nullptr;
     a.hashes = nullptr;
@@ -668,15 +705,11 @@ kdgrt_status kdgrt_render(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgrt
         return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render: unknown output format");
     ViewArgs a;
     fill_view_args(a, sc, cam, W, H, depth, out);
-    const dim3 grid(a.blocks_x * a.blocks_y), block(256);
+    const dim3 grid(interleave_grid<64>((uint64_t)a.blocks_x * a.blocks_y * 64)), block(64);
     hipStream_t s = (hipStream_t)stream;
-    if (depth <= 12) {
-        if (fmt == KDGRT_OUT_RGBT_F32) hipLaunchKernelGGL((k_view<12, 8, false, KDGRT_OUT_RGBT_F32>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((k_view<12, 8, false, KDGRT_OUT_RGB_F32>), grid, block, 0, s, a);
-    } else {
-        if (fmt == KDGRT_OUT_RGBT_F32) hipLaunchKernelGGL((k_view<32, 8, false, KDGRT_OUT_RGBT_F32>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((k_view<32, 8, false, KDGRT_OUT_RGB_F32>), grid, block, 0, s, a);
-    }
+    const size_t lds = stack_lds_bytes(depth, 64);
+    if (fmt == KDGRT_OUT_RGBT_F32) hipLaunchKernelGGL((k_view<8, false, KDGRT_OUT_RGBT_F32>), grid, block, lds, s, a);
+    else hipLaunchKernelGGL((k_view<8, false, KDGRT_OUT_RGB_F32>), grid, block, lds, s, a);
     HIP_TRY(hipGetLastError());
     return KDGRT_OK;
 }
@@ -693,10 +726,24 @@ kdgrt_status kdgrt_render_debug(kdgrt_context* ctx, const kdgrt_scene* sc, const
     a.hashes = hashes;
     a.seq = seq;
     a.seq_cap = seq ? seq_cap : 0;
-    const dim3 grid(a.blocks_x * a.blocks_y), block(256);
+    const dim3 grid(interleave_grid<64>((uint64_t)a.blocks_x * a.blocks_y * 64)), block(64);
     hipStream_t s = (hipStream_t)stream;
-    if (depth <= 12) hipLaunchKernelGGL((k_view<12, 8, true, KDGRT_OUT_RGBT_F32>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((k_view<32, 8, true, KDGRT_OUT_RGBT_F32>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((k_view<8, true, KDGRT_OUT_RGBT_F32>), grid, block, stack_lds_bytes(depth, 64), s, a);
+    HIP_TRY(hipGetLastError());
+    return KDGRT_OK;
+}
+
+kdgrt_status kdgrt_render_profile(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgrt_camera_info* cam, int W,
+                                  int H, float* out, uint64_t* phase_cycles, void* stream) {
+    kdgrt_status st = check_view(ctx, sc, cam, W, H, 12, out);
+    if (st != KDGRT_OK) return st;
+    if (!phase_cycles) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render_profile: null phase buffer");
+    ViewArgs a;
+    fill_view_args(a, sc, cam, W, H, 12, out);
+    a.prof = phase_cycles;
+    const dim3 grid(interleave_grid<64>((uint64_t)a.blocks_x * a.blocks_y * 64)), block(64);
+    hipLaunchKernelGGL((k_view<8, false, KDGRT_OUT_RGBT_F32, true>), grid, block, stack_lds_bytes(12, 64),
+                       (hipStream_t)stream, a);
     HIP_TRY(hipGetLastError());
     return KDGRT_OK;
 }
@@ -707,17 +754,13 @@ static kdgrt_status launch_rays(kdgrt_context* ctx, const kdgrt_scene* sc, const
     if (!ctx || !sc || (n && (!ro || !rd || !out))) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render_rays: null");
     if (depth < 1 || depth > 32) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render_rays: stack depth must be 1..32");
     if (n == 0) return KDGRT_OK;
-    if (n > (uint64_t)0x7FFFFFFF * 256ull) return set_error(KDGRT_ERR_CAPACITY, "render_rays: too many rays");
+    if (n > (uint64_t)0x7FFFF000 * 64ull) return set_error(KDGRT_ERR_CAPACITY, "render_rays: too many rays");
     RaysArgs a{sc->dev, ro, rd, n, depth, (float4*)out, counters, hashes};
-    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    const dim3 grid(interleave_grid<64>((n + 63) / 64)), block(64);
     hipStream_t s = (hipStream_t)stream;
-    if (dbg) {
-        if (depth <= 12) hipLaunchKernelGGL((k_rays<12, 8, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((k_rays<32, 8, true>), grid, block, 0, s, a);
-    } else {
-        if (depth <= 12) hipLaunchKernelGGL((k_rays<12, 8, false>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((k_rays<32, 8, false>), grid, block, 0, s, a);
-    }
+    const size_t lds = stack_lds_bytes(depth, 64);
+    if (dbg) hipLaunchKernelGGL((k_rays<8, true>), grid, block, lds, s, a);
+    else hipLaunchKernelGGL((k_rays<8, false>), grid, block, lds, s, a);
     HIP_TRY(hipGetLastError());
     return KDGRT_OK;
 }
@@ -763,8 +806,8 @@ kdgrt_status kdgrt_tuning_default(kdgrt_context* ctx, kdgrt_tuning* t) {
     t->tile_h = 8;
     t->stack_depth = 12;
     t->hit_capacity = 8;
-    t->waves_per_cu = 16;
-    t->block_threads = 256;
+    t->waves_per_cu = 32;
+    t->block_threads = 64;
     return KDGRT_OK;
 }
 
@@ -919,7 +962,7 @@ kdgrt_status batch_build(kdgrt_context* ctx, const kdgrt_view* views, int n_view
     a.depth = t.stack_depth;
     a.tile_w = t.tile_w;
     a.tile_h = t.tile_h;
-    b->kernel = pick_batch_kernel(t.stack_depth <= 12 ? 12 : 32, t.hit_capacity);
+    b->kernel = pick_batch_kernel(t.hit_capacity);
     b->block = t.block_threads;
     const int n_cu = ctx->prop.multiProcessorCount > 0 ? ctx->prop.multiProcessorCount : 256;
     const int waves = t.waves_per_cu * n_cu;
@@ -934,7 +977,8 @@ kdgrt_status batch_build(kdgrt_context* ctx, const kdgrt_view* views, int n_view
 kdgrt_status batch_launch(kdgrt_batch* b, hipStream_t s) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, b->d_counters, kXcds);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(b->kernel, dim3(b->grid), dim3(b->block), 0, s, b->args);
+    hipLaunchKernelGGL(b->kernel, dim3(b->grid), dim3(b->block), stack_lds_bytes(b->args.depth, b->block), s,
+                       b->args);
     HIP_TRY(hipGetLastError());
     return KDGRT_OK;
 }
@@ -1006,8 +1050,8 @@ kdgrt_status slot_launch(BatchSlot* sl, const kdgrt_view* views, int n, const kd
     int grid = std::max(1, t.waves_per_cu * n_cu * 64 / t.block_threads);
     grid = std::min<int>(grid, (int)std::max<uint64_t>(1, (tiles * 64 + t.block_threads - 1) / t.block_threads));
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, sl->d_counters, kXcds);
-    hipLaunchKernelGGL(pick_batch_kernel(t.stack_depth <= 12 ? 12 : 32, t.hit_capacity), dim3(grid),
-                       dim3(t.block_threads), 0, s, a);
+    hipLaunchKernelGGL(pick_batch_kernel(t.hit_capacity), dim3(grid), dim3(t.block_threads),
+                       stack_lds_bytes(t.stack_depth, t.block_threads), s, a);
     HIP_TRY(hipGetLastError());
     if (rays) *rays = nr;
     return KDGRT_OK;

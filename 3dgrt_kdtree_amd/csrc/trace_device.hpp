@@ -15,9 +15,10 @@
 // expf -> kd_expf, no FMA contraction (-ffp-contract=off).
 //
 // Machine plan (not the reference's):
-//  * the short stack is a register shift-register of DMAX entries holding
-//    the newest `depth` entries: push drops the oldest exactly like the
-//    reference's overwrite-when-full ring, pop returns the newest;
+//  * the short stack is the reference's ring (overwrite the oldest when
+//    full, restart from the root when empty) held in LDS, 8 B per entry,
+//    laid out entry-major across the wave's 64 lanes; push and pop are one
+//    LDS access each and cost no registers beyond the ring index;
 //  * the per-leaf hit list is a fixed-capacity register buffer of K records
 //    kept sorted by (t, leaf slot), which is the stable insertion-sort
 //    order; a leaf with more than K hits is finished by exact re-scans that
@@ -28,8 +29,9 @@
 //    for non-zero length, and zero length is rejected by the t test) and
 //    the opacity fetched with the geometry, so a rejected sample reads no
 //    SH data and an accepted one reads 192 B;
-//  * the 15 SH basis scalars depend only on the ray direction and are
-//    computed once per ray with the reference's double-precision products.
+//  * the 15 SH basis scalars depend only on the ray direction; they are
+//    formed with the reference's double-precision products when a sample is
+//    blended (about five times per ray) instead of occupying registers.
 #pragma once
 
 #include <hip/hip_fp16.h>
@@ -44,8 +46,9 @@ struct DevScene {
     const uint2* __restrict__ nodes;
     const uint32_t* __restrict__ offs;
     const float4* __restrict__ geom;  // 3 x float4 per Gaussian: pos|opacity, inv scale|k_scale, quat wxyz
-    const float4* __restrict__ shade; // 12 x float4 per Gaussian: f_dc[3], f_rest[45]
+    const float4* __restrict__ shade; // 12 x float4 per Gaussian, channel-major (f_dc[c], f_rest[15c..15c+14])
     const float4* __restrict__ sph;   // 1 x float4 per Gaussian: centre, packed (radius, kappa) as 2 x f16
+    const float4* __restrict__ lsph;  // the same records in leaf-slot order: lsph[k] = sph[offs[k]]
     float bmin[3];
     float bmax[3];
 };
@@ -61,7 +64,21 @@ struct RayDbg {
     uint32_t* seq;
     int seq_cap;
     int nseq;
+    // phase profile (PROF variant): shader-clock cycles between marks, by phase
+    // 0 descend, 1 pre-test windows, 2 exact tests + hit buffer, 3 composite,
+    // 4 stack pop / restart / loop control
+    uint64_t cyc[5];
+    uint64_t tlast;
 };
+
+#define KD_PROF_MARK(ph)                                                                                        \
+    do {                                                                                                        \
+        if (PROF) {                                                                                             \
+            const uint64_t now_ = clock64();                                                                    \
+            dbg->cyc[ph] += now_ - dbg->tlast;                                                                  \
+            dbg->tlast = now_;                                                                                  \
+        }                                                                                                       \
+    } while (0)
 
 // A validated tree visits each leaf at most once per ray; this cap is a
 // guard against malformed input only and is never reached otherwise.
@@ -114,26 +131,27 @@ __device__ __forceinline__ void sh_basis(const float d[3], float s[16]) {
 
 // Radiance of one Gaussian: the 16 coefficient float3s times the basis
 // scalars, accumulated in the reference's left-to-right order with the signs
-// of degree 1 (minus, plus, minus), +0.5, clamped below at 0.
+// of degree 1 (minus, plus, minus), +0.5, clamped below at 0.  The device SH
+// record is channel-major: channel c occupies float4s 4c..4c+3 as
+// (f_dc[c], f_rest[15c+0 .. 15c+14]), so one channel is loaded at a time.
 __device__ __forceinline__ void sh_radiance(const float4* __restrict__ sh, const float s[16], float rad[3]) {
-    float c[48];
-#pragma unroll
-    for (int i = 0; i < 12; ++i) {
-        const float4 q = sh[i];
-        c[4 * i + 0] = q.x;
-        c[4 * i + 1] = q.y;
-        c[4 * i + 2] = q.z;
-        c[4 * i + 3] = q.w;
-    }
-    // c[0..2] = f_dc, c[3 + 15*ch + i] = f_rest[15*ch + i]
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
-        float r = c[ch] * s[0];
-        r = r - c[3 + 15 * ch + 0] * s[1];
-        r = r + c[3 + 15 * ch + 1] * s[2];
-        r = r - c[3 + 15 * ch + 2] * s[3];
+        float c[16];
 #pragma unroll
-        for (int i = 3; i < 15; ++i) r = r + c[3 + 15 * ch + i] * s[i + 1];
+        for (int i = 0; i < 4; ++i) {
+            const float4 q = sh[4 * ch + i];
+            c[4 * i + 0] = q.x;
+            c[4 * i + 1] = q.y;
+            c[4 * i + 2] = q.z;
+            c[4 * i + 3] = q.w;
+        }
+        float r = c[0] * s[0];
+        r = r - c[1] * s[1];
+        r = r + c[2] * s[2];
+        r = r - c[3] * s[3];
+#pragma unroll
+        for (int i = 4; i < 16; ++i) r = r + c[i] * s[i];
         r = r + 0.5f;
         rad[ch] = fmaxf(r, 0.0f);
     }
@@ -166,11 +184,11 @@ struct TestOut {
     bool hit;
 };
 
-// rayPrimIntersect (cudaRenderer.cu:1069-1135) on the 48-byte geometry record.
-__device__ __forceinline__ TestOut ray_gaussian(const float o[3], const float d[3], const float4* __restrict__ g3,
-                                                float t_near, float t_far) {
+// rayPrimIntersect (cudaRenderer.cu:1069-1135) on the 48-byte geometry record
+// (a = pos|opacity, b = inverse scale|k_scale, q = quaternion w,x,y,z).
+__device__ __forceinline__ TestOut ray_gaussian(const float o[3], const float d[3], const float4 a, const float4 b,
+                                                const float4 q, float t_near, float t_far) {
     TestOut r;
-    const float4 a = g3[0], b = g3[1], q = g3[2];
     const float giscl[3] = {b.x, b.y, b.z};
     // quaternionWXYZToMatrixTranspose (cudaRenderer.cu:808-828)
     const float qr = q.x, qx = q.y, qy = q.z, qz = q.w;
@@ -218,9 +236,53 @@ __device__ __forceinline__ TestOut ray_gaussian(const float o[3], const float d[
     return r;
 }
 
-template <int DMAX, int K, bool DBG>
+// Inserts a hit into the K-entry register buffer kept sorted by (t, slot);
+// slots arrive in increasing order, so "after every entry with t <= r.t" is
+// the stable insertion-sort position.  A hit that does not fit (or evicts the
+// last entry) marks the buffer as overflowed.
+template <int K>
+__device__ __forceinline__ void insert_hit(float (&kt)[K], float (&kg)[K], float (&ko)[K], uint32_t (&ks)[K],
+                                           uint32_t (&kid)[K], int& n, bool& overflow, const TestOut& r, uint32_t s,
+                                           uint32_t gid) {
+    int pos = 0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) pos += (i < n && kt[i] <= r.t) ? 1 : 0;
+    if (pos >= K) {
+        overflow = true;
+        return;
+    }
+    if (n == K) overflow = true;
+#pragma unroll
+    for (int i = K - 1; i > 0; --i) {
+        if (i > pos) {
+            kt[i] = kt[i - 1];
+            kg[i] = kg[i - 1];
+            ko[i] = ko[i - 1];
+            ks[i] = ks[i - 1];
+            kid[i] = kid[i - 1];
+        } else if (i == pos) {
+            kt[i] = r.t;
+            kg[i] = r.gd;
+            ko[i] = r.op;
+            ks[i] = s;
+            kid[i] = gid;
+        }
+    }
+    if (pos == 0) {
+        kt[0] = r.t;
+        kg[0] = r.gd;
+        ko[0] = r.op;
+        ks[0] = s;
+        kid[0] = gid;
+    }
+    n = n + 1 < K ? n + 1 : K;
+}
+
+// stk: this lane's short-stack ring in LDS, entry e at stk[64 * e]
+// (entry-major over the 64 lanes of the wave), `depth` entries.
+template <int K, bool DBG, bool PROF = false>
 __device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], const float d[3], int depth,
-                                          float rgb[3], float& A, RayDbg* dbg) {
+                                          uint2* __restrict__ stk, float rgb[3], float& A, RayDbg* dbg) {
     rgb[0] = rgb[1] = rgb[2] = 0.0f;
     A = 0.0f;
     float t_near = 1e-2f, t_far = 3.402823466e+38f;
@@ -234,17 +296,12 @@ __device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], 
     if (!((t_far >= t_near) & (t_far >= 0.f))) return;
     const float t_scene_far = t_far;
 
-    float shs[16];
-    sh_basis(d, shs);
-
-    uint32_t stn[DMAX];
-    float stt[DMAX];
-#pragma unroll
-    for (int i = 0; i < DMAX; ++i) {
-        stn[i] = 0;
-        stt[i] = 0.f;
-    }
-    int quant = 0;
+    // ring stack state (cudaRenderer.cu:189-210): _top starts at depth-1
+    uint32_t top = (uint32_t)depth - 1u, quant = 0;
+    // Scalar copies for the per-node axis select: selecting between array
+    // elements lets the compiler turn the select into an indexed load and
+    // put the ray in scratch memory.
+    const float ox = o[0], oy = o[1], oz = o[2], dx = d[0], dy = d[1], dz = d[2];
 
     uint2 node = sc.nodes[0];
     int guard = 0;
@@ -252,25 +309,21 @@ __device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], 
         while ((node.x & 7u) != 3u) {
             const uint32_t child = node.x >> 3;
             const uint32_t axis = node.x & 3u;
-            const float oa = axis == 0 ? o[0] : (axis == 1 ? o[1] : o[2]);
-            const float da = axis == 0 ? d[0] : (axis == 1 ? d[1] : d[2]);
+            const float oa = axis == 0 ? ox : (axis == 1 ? oy : oz);
+            const float da = axis == 0 ? dx : (axis == 1 ? dy : dz);
             const float t_split = (__uint_as_float(node.y) - oa) / da;
             const uint32_t sign = __float_as_uint(da) >> 31;
             const uint32_t idx = child + (sign ^ (t_split <= t_near ? 1u : 0u));
             if (t_near < t_split && t_split < t_far) {
-#pragma unroll
-                for (int i = DMAX - 1; i > 0; --i) {
-                    stn[i] = stn[i - 1];
-                    stt[i] = stt[i - 1];
-                }
-                stn[0] = child + (sign ^ 1u);
-                stt[0] = t_far;
-                quant = quant + 1 < depth ? quant + 1 : depth;
+                if (++top == (uint32_t)depth) top = 0; // push overwrites the oldest when full
+                quant = quant + 1 < (uint32_t)depth ? quant + 1 : (uint32_t)depth;
+                stk[64 * top] = make_uint2(child + (sign ^ 1u), __float_as_uint(t_far));
                 t_far = t_split;
             }
             if (DBG) dbg->cnt[0]++;
             node = sc.nodes[idx];
         }
+        KD_PROF_MARK(0);
 
         // ---- leaf ----
         const uint32_t base = node.y, count = node.x >> 3;
@@ -284,13 +337,14 @@ __device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], 
         int first_hits = 0;
         while (true) {
             float kt[K], kg[K], ko[K];
-            uint32_t ks[K];
+            uint32_t ks[K], kid[K];
 #pragma unroll
             for (int i = 0; i < K; ++i) {
                 kt[i] = 0.f;
                 kg[i] = 0.f;
                 ko[i] = 0.f;
                 ks[i] = 0;
+                kid[i] = 0;
             }
             int n = 0;
             bool overflow = false;
@@ -303,54 +357,44 @@ __device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], 
                 const uint32_t wn = (count - w0) < 64u ? (count - w0) : 64u;
                 uint64_t mask = 0;
                 for (uint32_t j0 = 0; j0 < wn; j0 += 4) {
-                    uint32_t gids[4];
+                    // four leaf-ordered bound spheres per step, one contiguous read
+                    float4 sp[4];
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) gids[j] = (j0 + j < wn) ? sc.offs[base + w0 + j0 + j] : 0u;
+                    for (int j = 0; j < 4; ++j)
+                        sp[j] = (j0 + j < wn) ? sc.lsph[base + w0 + j0 + j] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
-                        if (j0 + j < wn && sphere_pass(o, d, sc.sph[gids[j]], t_near, t_far))
-                            mask |= 1ull << (j0 + j);
+                        if (j0 + j < wn && sphere_pass(o, d, sp[j], t_near, t_far)) mask |= 1ull << (j0 + j);
                     }
                 }
+                KD_PROF_MARK(1);
+                // exact tests, two survivors per step so their loads overlap
                 while (mask) {
-                    const uint32_t j = (uint32_t)__builtin_ctzll(mask);
+                    const uint32_t j1 = (uint32_t)__builtin_ctzll(mask);
                     mask &= mask - 1;
-                    const uint32_t s = w0 + j;
-                    const uint32_t gid = sc.offs[base + s];
-                    const TestOut r = ray_gaussian(o, d, sc.geom + 3 * (size_t)gid, t_near, t_far);
-                    if (!r.hit) continue;
-                    if (have_lb && !(r.t > lb_t || (r.t == lb_t && s > lb_s))) continue;
-                    if (DBG && !have_lb) first_hits++;
-                    int pos = 0;
+                    const bool two = mask != 0;
+                    const uint32_t j2 = two ? (uint32_t)__builtin_ctzll(mask) : j1;
+                    if (two) mask &= mask - 1;
+                    const uint32_t s1 = w0 + j1, s2 = w0 + j2;
+                    const uint32_t g1 = sc.offs[base + s1];
+                    const uint32_t g2 = two ? sc.offs[base + s2] : g1;
+                    const float4* r1p = sc.geom + 3 * (size_t)g1;
+                    const float4* r2p = sc.geom + 3 * (size_t)g2;
+                    const float4 a1 = r1p[0], b1 = r1p[1], q1 = r1p[2];
+                    const float4 a2 = r2p[0], b2 = r2p[1], q2 = r2p[2];
 #pragma unroll
-                    for (int i = 0; i < K; ++i) pos += (i < n && kt[i] <= r.t) ? 1 : 0;
-                    if (pos >= K) {
-                        overflow = true;
-                        continue;
+                    for (int u = 0; u < 2; ++u) {
+                        if (u == 1 && !two) break;
+                        const uint32_t s = u ? s2 : s1, gid = u ? g2 : g1;
+                        const TestOut r = u ? ray_gaussian(o, d, a2, b2, q2, t_near, t_far)
+                                            : ray_gaussian(o, d, a1, b1, q1, t_near, t_far);
+                        if (!r.hit) continue;
+                        if (have_lb && !(r.t > lb_t || (r.t == lb_t && s > lb_s))) continue;
+                        if (DBG && !have_lb) first_hits++;
+                        insert_hit<K>(kt, kg, ko, ks, kid, n, overflow, r, s, gid);
                     }
-                    if (n == K) overflow = true;
-#pragma unroll
-                    for (int i = K - 1; i > 0; --i) {
-                        if (i > pos) {
-                            kt[i] = kt[i - 1];
-                            kg[i] = kg[i - 1];
-                            ko[i] = ko[i - 1];
-                            ks[i] = ks[i - 1];
-                        } else if (i == pos) {
-                            kt[i] = r.t;
-                            kg[i] = r.gd;
-                            ko[i] = r.op;
-                            ks[i] = s;
-                        }
-                    }
-                    if (pos == 0) {
-                        kt[0] = r.t;
-                        kg[0] = r.gd;
-                        ko[0] = r.op;
-                        ks[0] = s;
-                    }
-                    n = n + 1 < K ? n + 1 : K;
                 }
+                KD_PROF_MARK(2);
             }
             if (DBG && !have_lb) {
                 dbg->cnt[3] += first_hits;
@@ -364,7 +408,14 @@ __device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], 
                 const float gres = kd_expf(-0.0555555555556f * gd * gd);
                 const float alpha = fminf(0.99f, ko[i] * gres);
                 if (gres < 0.0113f || alpha < 1.0f / 255.0f) continue;
-                const uint32_t gid = sc.offs[base + ks[i]];
+                const uint32_t gid = kid[i];
+                // The basis depends only on the ray direction; it is formed here,
+                // per blended sample, rather than held in 16 registers per ray.
+                float bx = dx, by = dy, bz = dz;
+                asm volatile("" : "+v"(bx), "+v"(by), "+v"(bz));
+                const float dv[3] = {bx, by, bz};
+                float shs[16];
+                sh_basis(dv, shs);
                 float c[3];
                 sh_radiance(sc.shade + 12 * (size_t)gid, shs, c);
                 const float w = 1.0f - A;
@@ -380,6 +431,7 @@ __device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], 
                 }
                 if (A > 0.99f) done = true;
             }
+            KD_PROF_MARK(3);
             if (done || !overflow) break;
             lb_t = kt[K - 1];
             lb_s = ks[K - 1];
@@ -393,19 +445,16 @@ __device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], 
             t_far = t_scene_far;
             if (DBG) dbg->cnt[7]++;
         } else {
-            const uint32_t nid = stn[0];
-            const float tm = stt[0];
-#pragma unroll
-            for (int i = 0; i < DMAX - 1; ++i) {
-                stn[i] = stn[i + 1];
-                stt[i] = stt[i + 1];
-            }
+            const uint2 e = stk[64 * top];
+            if (top == 0) top = (uint32_t)depth;
+            --top;
             --quant;
-            node = sc.nodes[nid];
+            node = sc.nodes[e.x];
             t_near = t_far;
-            t_far = tm;
+            t_far = __uint_as_float(e.y);
         }
         if (++guard > kLeafGuard) break;
+        KD_PROF_MARK(4);
     }
 }
 

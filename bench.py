@@ -117,6 +117,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--phases", action="store_true", help="add the per-phase cycle profile of the plain kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); this build has no multi-GPU path.",
@@ -162,6 +163,21 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": load_traffic("plain_1080p_1M"),
             "algorithmic_bytes_per_launch": B, "bytes_per_ray": round(B / rays, 1), "kernel": "k_view<12,8,false,0>"}
 
+    phases = None
+    if args.phases:
+        import ctypes as C
+        prof = dev.alloc(W * H * 5 * 8)
+        K.check(K.lib().kdgrt_render_profile(dev.h, sc.h, C.byref(ci), C.c_int(W), C.c_int(H), out.ptr, prof.ptr,
+                                             None))
+        dev.sync()
+        cyc = prof.download(np.uint64, (H * W, 5)).astype(np.float64)
+        prof.free()
+        names = ["descend", "pretest", "exact_insert", "composite", "pop_loop"]
+        tot = cyc.sum()
+        phases = {n: round(float(cyc[:, i].sum() / tot), 4) for i, n in enumerate(names)}
+        phases["mean_cycles_per_ray"] = round(float(cyc.sum(1).mean()), 1)
+        log(f"phase shares: {phases}")
+
     result = {"metric": METRIC, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
               "warmup": args.warmup, "ms_per_step": round(ms_launch, 4), "higher_is_better": True, "scaling": "weak",
               "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -169,6 +185,7 @@ def main():
                                      f"view from radius 25, 1 ray/pixel, short stack 12",
                          "gaussians": len(g), "width": W, "height": H, "stack_depth": 12},
               "device": info["name"], "roofline": roof,
+              **({"phases": phases} if phases else {}),
               "build": {"host_sah_build_s": round(build_s, 2), "threads": os.cpu_count(), "nodes": st["n_nodes"],
                         "leaf_refs": st["n_offsets"], "proxy_triangles": st["n_triangles"]}}
 

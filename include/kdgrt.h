@@ -213,6 +213,14 @@ KDGRT_API kdgrt_status kdgrt_render_debug(kdgrt_context* ctx, const kdgrt_scene*
                                           const kdgrt_camera_info* cam, int width, int height, int stack_depth,
                                           float* out_rgbt, int32_t* counters, uint64_t* id_hash,
                                           uint32_t* id_seq, int seq_cap, void* stream);
+/* Measurement helper, no reference counterpart: the plain kernel with
+ * shader-clock marks, writing per ray (y*width+x) five uint64 cycle sums:
+ * descend, pre-test windows, exact tests + hit buffer, composite, stack
+ * pop/restart/loop control (time a lane spends masked off is charged to
+ * its next mark). */
+KDGRT_API kdgrt_status kdgrt_render_profile(kdgrt_context* ctx, const kdgrt_scene* scene,
+                                            const kdgrt_camera_info* cam, int width, int height, float* out_rgbt,
+                                            uint64_t* phase_cycles, void* stream);
 /* Explicit rays (device float3 arrays, directions used as given). */
 KDGRT_API kdgrt_status kdgrt_render_rays(kdgrt_context* ctx, const kdgrt_scene* scene, const float* origins,
                                          const float* dirs, uint64_t n_rays, int stack_depth, float* out_rgbt,
