@@ -21,7 +21,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libkdgrt.so")
+LIB_PATH = os.environ.get("KDGRT_LIB_PATH") or os.path.join(_HERE, "lib", "libkdgrt.so")
 
 GAUSSIAN_DTYPE = np.dtype([
     ("pos", "<f4", 3), ("opacity", "<f4"), ("scale", "<f4", 3), ("k_scale", "<f4"), ("rot", "<f4", 4),

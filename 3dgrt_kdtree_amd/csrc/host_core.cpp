@@ -463,6 +463,21 @@ kdgrt_status validate_tree(const kdgrt_node* nodes, uint32_t n_nodes, uint64_t n
                 return set_error(KDGRT_ERR_FORMAT, "tree: leaf GID " + std::to_string(offsets[k]) + " >= " +
                                                        std::to_string(n_gaussians));
     }
+    if (offsets) {
+        // Each leaf lists a Gaussian at most once (the reference's leaf lists
+        // are std::set contents, Kd-treeConstructor.cpp:1543-1586).
+        std::vector<uint32_t> ids;
+        for (uint32_t i = 0; i < n_nodes; ++i) {
+            const kdgrt_node nd = nodes[i];
+            if ((nd.x & 7u) != 3u) continue;
+            const uint32_t cnt = nd.x >> 3;
+            if (cnt < 2) continue;
+            ids.assign(offsets + nd.y, offsets + nd.y + cnt);
+            std::sort(ids.begin(), ids.end());
+            if (std::adjacent_find(ids.begin(), ids.end()) != ids.end())
+                return set_error(KDGRT_ERR_FORMAT, "tree: leaf " + std::to_string(i) + " lists a Gaussian twice");
+        }
+    }
     return KDGRT_OK;
 }
 

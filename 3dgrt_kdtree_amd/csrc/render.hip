@@ -91,12 +91,17 @@ __global__ __launch_bounds__(64) void k_view(ViewArgs a) {
     }
     if (PROF) {
         for (int i = 0; i < 5; ++i) dbg.cyc[i] = 0;
+        dbg.pretests = dbg.survivors = 0;
         dbg.tlast = clock64();
     }
     extern __shared__ uint2 lds_stack[];
     trace_ray<K, DBG, PROF>(a.sc, o, d, a.depth, lds_stack + lane, rgb, A, &dbg);
-    if (PROF)
-        for (int i = 0; i < 5; ++i) a.prof[5 * rid + i] = dbg.cyc[i];
+    if (PROF) {
+        for (int i = 0; i < 5; ++i) a.prof[8 * rid + i] = dbg.cyc[i];
+        a.prof[8 * rid + 5] = dbg.pretests;
+        a.prof[8 * rid + 6] = dbg.survivors;
+        a.prof[8 * rid + 7] = 0;
+    }
     const size_t pid = (size_t)(a.H - y - 1) * a.W + x; // framebuffer row flip (cudaRenderer.cu:2367)
     if (FMT == KDGRT_OUT_RGBT_F32) {
         reinterpret_cast<float4*>(a.out)[pid] = make_float4(rgb[0], rgb[1], rgb[2], 1.0f - A);
@@ -374,14 +379,26 @@ static uint16_t f32_to_f16_up(float x) {
     return __half_as_ushort(h);
 }
 
-static float4 bound_sphere(const float4& a, const float4& b, const float4& q) {
+// quaternionWXYZToMatrixTranspose (cudaRenderer.cu:808-828), f32, q = (w, x, y, z).
+static void quat_to_mt(const float4& q, float m[3][3]) {
     const float qr = q.x, qx = q.y, qy = q.z, qz = q.w;
     const float xx = qx * qx, yy = qy * qy, zz = qz * qz;
     const float xy = qx * qy, xz = qx * qz, yz = qy * qz;
     const float rx = qr * qx, ry = qr * qy, rz = qr * qz;
-    const float mf[3][3] = {{(1.f - 2.f * (yy + zz)), 2.f * (xy + rz), 2.f * (xz - ry)},
-                            {2.f * (xy - rz), (1.f - 2.f * (xx + zz)), 2.f * (yz + rx)},
-                            {2.f * (xz + ry), 2.f * (yz - rx), (1.f - 2.f * (xx + yy))}};
+    m[0][0] = (1.f - 2.f * (yy + zz));
+    m[0][1] = 2.f * (xy + rz);
+    m[0][2] = 2.f * (xz - ry);
+    m[1][0] = 2.f * (xy - rz);
+    m[1][1] = (1.f - 2.f * (xx + zz));
+    m[1][2] = 2.f * (yz + rx);
+    m[2][0] = 2.f * (xz + ry);
+    m[2][1] = 2.f * (yz - rx);
+    m[2][2] = (1.f - 2.f * (xx + yy));
+}
+
+static float4 bound_sphere(const float4& a, const float4& b, const float4& q) {
+    float mf[3][3];
+    quat_to_mt(q, mf);
     const double g[3] = {b.x, b.y, b.z};
     double r = INFINITY, kap = 0.0;
     bool ok = std::isfinite(a.x) && std::isfinite(a.y) && std::isfinite(a.z);
@@ -556,7 +573,7 @@ kdgrt_status kdgrt_copy_f32x4(kdgrt_context* ctx, void* dst, const void* src, ui
 
 uint64_t kdgrt_scene_bytes_estimate(uint32_t n_nodes, uint64_t n_offsets, uint64_t n_gaussians) {
     auto r = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
-    return r(8ull * n_nodes) + r(4ull * n_offsets) + r(48ull * n_gaussians) + r(192ull * n_gaussians) +
+    return r(8ull * n_nodes) + r(4ull * n_offsets) + r(80ull * n_gaussians) + r(192ull * n_gaussians) +
            r(16ull * n_gaussians) + r(16ull * n_offsets);
 }
 
@@ -571,7 +588,7 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
     HIP_TRY(hipSetDevice(ctx->device));
     // Repack: 48-B geometry records (pos|opacity, inverse scale|k_scale,
     // quaternion) and 192-B SH records.
-    std::vector<float4> geom(3 * n_g), shade(12 * n_g), sph(n_g), lsph(std::max<uint64_t>(1, n_offsets));
+    std::vector<float4> geom(5 * n_g), shade(12 * n_g), sph(n_g), lsph(std::max<uint64_t>(1, n_offsets));
     for (uint64_t i = 0; i < n_g; ++i) {
         const kdgrt_gaussian& s = g[i];
         float inv[3], kinv;
@@ -586,15 +603,28 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
             inv[2] = 1 / s.scale[2];
             kinv = 1 / s.k_scale;
         }
-        geom[3 * i + 0] = make_float4(s.pos[0], s.pos[1], s.pos[2], s.opacity);
-        geom[3 * i + 1] = make_float4(inv[0], inv[1], inv[2], kinv);
-        geom[3 * i + 2] = make_float4(s.rot[0], s.rot[1], s.rot[2], s.rot[3]);
+        // Geometry record: pos|opacity, inverse scale|pscale.x, pscale.y,
+        // pscale.z, M00, M01 | M02, M10, M11, M12 | M20, M21, M22, 0 with
+        // pscale = 1 / (inverse scale) and M the transposed rotation of the
+        // quaternion, in the device's f32 operations (cudaRenderer.cu:808-828,
+        // 1090): computed once here instead of per test.
+        const float4 q = make_float4(s.rot[0], s.rot[1], s.rot[2], s.rot[3]);
+        float m[3][3];
+        quat_to_mt(q, m);
+        const float ps[3] = {1 / inv[0], 1 / inv[1], 1 / inv[2]};
+        geom[5 * i + 0] = make_float4(s.pos[0], s.pos[1], s.pos[2], s.opacity);
+        geom[5 * i + 1] = make_float4(inv[0], inv[1], inv[2], ps[0]);
+        geom[5 * i + 2] = make_float4(ps[1], ps[2], m[0][0], m[0][1]);
+        geom[5 * i + 3] = make_float4(m[0][2], m[1][0], m[1][1], m[1][2]);
+        geom[5 * i + 4] = make_float4(m[2][0], m[2][1], m[2][2], 0.0f);
+        (void)kinv;
         float* sh = reinterpret_cast<float*>(&shade[12 * i]); // channel-major SH record
         for (int ch = 0; ch < 3; ++ch) {
             sh[16 * ch] = s.f_dc[ch];
             for (int k = 0; k < 15; ++k) sh[16 * ch + 1 + k] = s.f_rest[15 * ch + k];
         }
-        sph[i] = bound_sphere(geom[3 * i + 0], geom[3 * i + 1], geom[3 * i + 2]);
+        sph[i] = bound_sphere(make_float4(s.pos[0], s.pos[1], s.pos[2], s.opacity),
+                              make_float4(inv[0], inv[1], inv[2], kinv), q);
     }
     for (uint64_t k = 0; k < n_offsets; ++k) lsph[k] = sph[offsets[k]]; // leaf-slot order
     auto* sc = new kdgrt_scene();
@@ -613,7 +643,7 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
     hipError_t e;
     if ((e = hipMalloc(&sc->d_nodes, 8ull * n_nodes)) != hipSuccess) return fail(e, "nodes");
     if ((e = hipMalloc(&sc->d_offs, std::max<uint64_t>(4, 4ull * n_offsets))) != hipSuccess) return fail(e, "offsets");
-    if ((e = hipMalloc(&sc->d_geom, 48ull * n_g)) != hipSuccess) return fail(e, "geometry");
+    if ((e = hipMalloc(&sc->d_geom, 80ull * n_g)) != hipSuccess) return fail(e, "geometry");
     if ((e = hipMalloc(&sc->d_shade, 192ull * n_g)) != hipSuccess) return fail(e, "sh");
     if ((e = hipMalloc(&sc->d_sph, 16ull * n_g)) != hipSuccess) return fail(e, "spheres");
     if ((e = hipMalloc(&sc->d_lsph, 16ull * lsph.size())) != hipSuccess) return fail(e, "leaf spheres");
@@ -624,7 +654,7 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
     if ((e = hipMemcpy(sc->d_nodes, nodes, 8ull * n_nodes, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "copy");
     if (n_offsets && (e = hipMemcpy(sc->d_offs, offsets, 4ull * n_offsets, hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "copy");
-    if ((e = hipMemcpy(sc->d_geom, geom.data(), 48ull * n_g, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "copy");
+    if ((e = hipMemcpy(sc->d_geom, geom.data(), 80ull * n_g, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "copy");
     if ((e = hipMemcpy(sc->d_shade, shade.data(), 192ull * n_g, hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "copy");
     sc->dev.nodes = (const uint2*)sc->d_nodes;
@@ -740,7 +770,7 @@ kdgrt_status kdgrt_render_profile(kdgrt_context* ctx, const kdgrt_scene* sc, con
     if (!phase_cycles) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render_profile: null phase buffer");
     ViewArgs a;
     fill_view_args(a, sc, cam, W, H, 12, out);
-    a.prof = phase_cycles;
+    a.prof = phase_cycles; // 8 x u64 per ray
     const dim3 grid(interleave_grid<64>((uint64_t)a.blocks_x * a.blocks_y * 64)), block(64);
     hipLaunchKernelGGL((k_view<8, false, KDGRT_OUT_RGBT_F32, true>), grid, block, stack_lds_bytes(12, 64),
                        (hipStream_t)stream, a);

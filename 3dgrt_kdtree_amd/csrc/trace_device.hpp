@@ -27,8 +27,9 @@
 //  * the composite step reuses the test's grayDist (bit-identical to the
 //    reference's recomputation: same operands, normalize == safe_normalize
 //    for non-zero length, and zero length is rejected by the t test) and
-//    the opacity fetched with the geometry, so a rejected sample reads no
-//    SH data and an accepted one reads 192 B;
+//    the opacity fetched with the geometry; a buffer entry is (t, grayDist,
+//    opacity, ID), a skipped sample reads no SH data and a blended one
+//    reads 192 B;
 //  * the 15 SH basis scalars depend only on the ray direction; they are
 //    formed with the reference's double-precision products when a sample is
 //    blended (about five times per ray) instead of occupying registers.
@@ -45,7 +46,7 @@ namespace kdgrt_dev {
 struct DevScene {
     const uint2* __restrict__ nodes;
     const uint32_t* __restrict__ offs;
-    const float4* __restrict__ geom;  // 3 x float4 per Gaussian: pos|opacity, inv scale|k_scale, quat wxyz
+    const float4* __restrict__ geom;  // 5 x float4 per Gaussian: pos|opacity, inverse scale, 1/inverse scale, M
     const float4* __restrict__ shade; // 12 x float4 per Gaussian, channel-major (f_dc[c], f_rest[15c..15c+14])
     const float4* __restrict__ sph;   // 1 x float4 per Gaussian: centre, packed (radius, kappa) as 2 x f16
     const float4* __restrict__ lsph;  // the same records in leaf-slot order: lsph[k] = sph[offs[k]]
@@ -69,6 +70,7 @@ struct RayDbg {
     // 4 stack pop / restart / loop control
     uint64_t cyc[5];
     uint64_t tlast;
+    uint32_t pretests, survivors; // PROF variant: candidates pre-tested / passed on to the exact test
 };
 
 #define KD_PROF_MARK(ph)                                                                                        \
@@ -184,21 +186,25 @@ struct TestOut {
     bool hit;
 };
 
-// rayPrimIntersect (cudaRenderer.cu:1069-1135) on the 48-byte geometry record
-// (a = pos|opacity, b = inverse scale|k_scale, q = quaternion w,x,y,z).
-__device__ __forceinline__ TestOut ray_gaussian(const float o[3], const float d[3], const float4 a, const float4 b,
-                                                const float4 q, float t_near, float t_far) {
+// rayPrimIntersect (cudaRenderer.cu:1069-1135) on the 80-byte geometry
+// record: g0 = pos|opacity, g1 = inverse scale|pscale.x, g2 = pscale.y,
+// pscale.z, M00, M01, g3 = M02, M10, M11, M12, g4 = M20, M21, M22, 0, where
+// M = quaternionWXYZToMatrixTranspose(rot) (cudaRenderer.cu:808-828) and
+// pscale = 1 / (inverse scale), both formed on the host with the same f32
+// operations.  grayDist < 8 is tested before t: a hit needs both, so the
+// order changes no outcome and most non-hits skip the t square root.
+__device__ __forceinline__ TestOut ray_gaussian(const float o[3], const float d[3], const float4 g0, const float4 g1,
+                                                const float4 g2, const float4 g3, const float4 g4, float t_near,
+                                                float t_far) {
     TestOut r;
-    const float giscl[3] = {b.x, b.y, b.z};
-    // quaternionWXYZToMatrixTranspose (cudaRenderer.cu:808-828)
-    const float qr = q.x, qx = q.y, qy = q.z, qz = q.w;
-    const float xx = qx * qx, yy = qy * qy, zz = qz * qz;
-    const float xy = qx * qy, xz = qx * qz, yz = qy * qz;
-    const float rx = qr * qx, ry = qr * qy, rz = qr * qz;
-    const float m[3][3] = {{(1.f - 2.f * (yy + zz)), 2.f * (xy + rz), 2.f * (xz - ry)},
-                           {2.f * (xy - rz), (1.f - 2.f * (xx + zz)), 2.f * (yz + rx)},
-                           {2.f * (xz + ry), 2.f * (yz - rx), (1.f - 2.f * (xx + yy))}};
-    const float gposc[3] = {o[0] - a.x, o[1] - a.y, o[2] - a.z};
+    r.hit = false;
+    r.t = 0.0f;
+    r.gd = 0.0f;
+    r.op = g0.w;
+    const float giscl[3] = {g1.x, g1.y, g1.z};
+    const float pscale[3] = {g1.w, g2.x, g2.y};
+    const float m[3][3] = {{g2.z, g2.w, g3.x}, {g3.y, g3.z, g3.w}, {g4.x, g4.y, g4.z}};
+    const float gposc[3] = {o[0] - g0.x, o[1] - g0.y, o[2] - g0.z};
     float gro[3], grdu[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -217,33 +223,31 @@ __device__ __forceinline__ TestOut ray_gaussian(const float o[3], const float d[
 #pragma unroll
         for (int k = 0; k < 3; ++k) grd[k] = grdu[k];
     }
-    const float grp = -(grd[0] * gro[0] + grd[1] * gro[1] + grd[2] * gro[2]);
-    float grds[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) grds[k] = ((1.0f / giscl[k]) * grd[k]) * grp;
-    const float t = (grp < 0.f ? -1.f : 1.f) * sqrtf(grds[0] * grds[0] + grds[1] * grds[1] + grds[2] * grds[2]);
-    r.t = t;
-    r.op = a.w;
-    r.hit = false;
-    r.gd = 0.0f;
-    if (t < t_near || t > t_far) return r;
     const float c0 = grd[1] * gro[2] - grd[2] * gro[1];
     const float c1 = grd[2] * gro[0] - grd[0] * gro[2];
     const float c2 = grd[0] * gro[1] - grd[1] * gro[0];
     const float gd = c0 * c0 + c1 * c1 + c2 * c2;
+    if (!(gd < 8.f)) return r;
+    const float grp = -(grd[0] * gro[0] + grd[1] * gro[1] + grd[2] * gro[2]);
+    float grds[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) grds[k] = (pscale[k] * grd[k]) * grp;
+    const float t = (grp < 0.f ? -1.f : 1.f) * sqrtf(grds[0] * grds[0] + grds[1] * grds[1] + grds[2] * grds[2]);
+    if (t < t_near || t > t_far) return r;
+    r.t = t;
     r.gd = gd;
-    r.hit = gd < 8.f;
+    r.hit = true;
     return r;
 }
 
-// Inserts a hit into the K-entry register buffer kept sorted by (t, slot);
+// Inserts a hit into the K-entry register buffer kept sorted by (t, slot):
 // slots arrive in increasing order, so "after every entry with t <= r.t" is
-// the stable insertion-sort position.  A hit that does not fit (or evicts the
-// last entry) marks the buffer as overflowed.
+// the stable insertion-sort position.  An entry holds t, grayDist, opacity
+// and the Gaussian ID.  A hit that does not fit (or evicts the last entry)
+// marks the buffer as overflowed.
 template <int K>
-__device__ __forceinline__ void insert_hit(float (&kt)[K], float (&kg)[K], float (&ko)[K], uint32_t (&ks)[K],
-                                           uint32_t (&kid)[K], int& n, bool& overflow, const TestOut& r, uint32_t s,
-                                           uint32_t gid) {
+__device__ __forceinline__ void insert_hit(float (&kt)[K], float (&kg)[K], float (&ko)[K], uint32_t (&kid)[K],
+                                           int& n, bool& overflow, const TestOut& r, uint32_t gid) {
     int pos = 0;
 #pragma unroll
     for (int i = 0; i < K; ++i) pos += (i < n && kt[i] <= r.t) ? 1 : 0;
@@ -258,13 +262,11 @@ __device__ __forceinline__ void insert_hit(float (&kt)[K], float (&kg)[K], float
             kt[i] = kt[i - 1];
             kg[i] = kg[i - 1];
             ko[i] = ko[i - 1];
-            ks[i] = ks[i - 1];
             kid[i] = kid[i - 1];
         } else if (i == pos) {
             kt[i] = r.t;
             kg[i] = r.gd;
             ko[i] = r.op;
-            ks[i] = s;
             kid[i] = gid;
         }
     }
@@ -272,7 +274,6 @@ __device__ __forceinline__ void insert_hit(float (&kt)[K], float (&kg)[K], float
         kt[0] = r.t;
         kg[0] = r.gd;
         ko[0] = r.op;
-        ks[0] = s;
         kid[0] = gid;
     }
     n = n + 1 < K ? n + 1 : K;
@@ -337,13 +338,12 @@ __device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], 
         int first_hits = 0;
         while (true) {
             float kt[K], kg[K], ko[K];
-            uint32_t ks[K], kid[K];
+            uint32_t kid[K];
 #pragma unroll
             for (int i = 0; i < K; ++i) {
                 kt[i] = 0.f;
                 kg[i] = 0.f;
                 ko[i] = 0.f;
-                ks[i] = 0;
                 kid[i] = 0;
             }
             int n = 0;
@@ -365,9 +365,11 @@ __device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], 
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         if (j0 + j < wn && sphere_pass(o, d, sp[j], t_near, t_far)) mask |= 1ull << (j0 + j);
+                        if (PROF && j0 + j < wn) dbg->pretests++;
                     }
                 }
                 KD_PROF_MARK(1);
+                if (PROF) dbg->survivors += (uint32_t)__builtin_popcountll(mask);
                 // exact tests, two survivors per step so their loads overlap
                 while (mask) {
                     const uint32_t j1 = (uint32_t)__builtin_ctzll(mask);
@@ -378,20 +380,20 @@ __device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], 
                     const uint32_t s1 = w0 + j1, s2 = w0 + j2;
                     const uint32_t g1 = sc.offs[base + s1];
                     const uint32_t g2 = two ? sc.offs[base + s2] : g1;
-                    const float4* r1p = sc.geom + 3 * (size_t)g1;
-                    const float4* r2p = sc.geom + 3 * (size_t)g2;
-                    const float4 a1 = r1p[0], b1 = r1p[1], q1 = r1p[2];
-                    const float4 a2 = r2p[0], b2 = r2p[1], q2 = r2p[2];
+                    const float4* r1p = sc.geom + 5 * (size_t)g1;
+                    const float4* r2p = sc.geom + 5 * (size_t)g2;
+                    const float4 a1 = r1p[0], b1 = r1p[1], c1 = r1p[2], e1 = r1p[3], f1 = r1p[4];
+                    const float4 a2 = r2p[0], b2 = r2p[1], c2 = r2p[2], e2 = r2p[3], f2 = r2p[4];
 #pragma unroll
                     for (int u = 0; u < 2; ++u) {
                         if (u == 1 && !two) break;
                         const uint32_t s = u ? s2 : s1, gid = u ? g2 : g1;
-                        const TestOut r = u ? ray_gaussian(o, d, a2, b2, q2, t_near, t_far)
-                                            : ray_gaussian(o, d, a1, b1, q1, t_near, t_far);
+                        const TestOut r = u ? ray_gaussian(o, d, a2, b2, c2, e2, f2, t_near, t_far)
+                                            : ray_gaussian(o, d, a1, b1, c1, e1, f1, t_near, t_far);
                         if (!r.hit) continue;
                         if (have_lb && !(r.t > lb_t || (r.t == lb_t && s > lb_s))) continue;
                         if (DBG && !have_lb) first_hits++;
-                        insert_hit<K>(kt, kg, ko, ks, kid, n, overflow, r, s, gid);
+                        insert_hit<K>(kt, kg, ko, kid, n, overflow, r, gid);
                     }
                 }
                 KD_PROF_MARK(2);
@@ -404,6 +406,7 @@ __device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], 
             for (int i = 0; i < K; ++i) {
                 if (i >= n || done) break;
                 if (DBG) dbg->cnt[6]++;
+                // particleResponse<4> and the skip rule (cudaRenderer.cu:1655-1671)
                 const float gd = kg[i];
                 const float gres = kd_expf(-0.0555555555556f * gd * gd);
                 const float alpha = fminf(0.99f, ko[i] * gres);
@@ -433,8 +436,11 @@ __device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], 
             }
             KD_PROF_MARK(3);
             if (done || !overflow) break;
+            // Re-scan above the last composited key (t, slot).  Leaf ID lists
+            // hold each ID once (validated), so the slot is the ID's position.
             lb_t = kt[K - 1];
-            lb_s = ks[K - 1];
+            lb_s = 0;
+            while (lb_s < count && sc.offs[base + lb_s] != kid[K - 1]) ++lb_s;
             have_lb = true;
         }
 

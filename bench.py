@@ -166,16 +166,19 @@ def main():
     phases = None
     if args.phases:
         import ctypes as C
-        prof = dev.alloc(W * H * 5 * 8)
+        prof = dev.alloc(W * H * 8 * 8)
         K.check(K.lib().kdgrt_render_profile(dev.h, sc.h, C.byref(ci), C.c_int(W), C.c_int(H), out.ptr, prof.ptr,
                                              None))
         dev.sync()
-        cyc = prof.download(np.uint64, (H * W, 5)).astype(np.float64)
+        pr = prof.download(np.uint64, (H * W, 8)).astype(np.float64)
         prof.free()
+        cyc = pr[:, :5]
         names = ["descend", "pretest", "exact_insert", "composite", "pop_loop"]
         tot = cyc.sum()
         phases = {n: round(float(cyc[:, i].sum() / tot), 4) for i, n in enumerate(names)}
         phases["mean_cycles_per_ray"] = round(float(cyc.sum(1).mean()), 1)
+        phases["pretests_per_ray"] = round(float(pr[:, 5].mean()), 2)
+        phases["exact_tests_per_ray"] = round(float(pr[:, 6].mean()), 2)
         log(f"phase shares: {phases}")
 
     result = {"metric": METRIC, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,

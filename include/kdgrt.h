@@ -214,10 +214,11 @@ KDGRT_API kdgrt_status kdgrt_render_debug(kdgrt_context* ctx, const kdgrt_scene*
                                           float* out_rgbt, int32_t* counters, uint64_t* id_hash,
                                           uint32_t* id_seq, int seq_cap, void* stream);
 /* Measurement helper, no reference counterpart: the plain kernel with
- * shader-clock marks, writing per ray (y*width+x) five uint64 cycle sums:
- * descend, pre-test windows, exact tests + hit buffer, composite, stack
- * pop/restart/loop control (time a lane spends masked off is charged to
- * its next mark). */
+ * shader-clock marks, writing per ray (y*width+x) eight uint64: cycle sums
+ * for descend, pre-test windows, exact tests + hit buffer, composite, stack
+ * pop/restart/loop control (time a lane spends masked off is charged to its
+ * next mark), then the number of pre-tested candidates, the number passed on
+ * to the exact test, and 0. */
 KDGRT_API kdgrt_status kdgrt_render_profile(kdgrt_context* ctx, const kdgrt_scene* scene,
                                             const kdgrt_camera_info* cam, int width, int height, float* out_rgbt,
                                             uint64_t* phase_cycles, void* stream);
