@@ -31,6 +31,11 @@ namespace {
 
 constexpr int kXcds = 8;
 
+// Hit-buffer capacity of the plain view and ray-list launches (0.013 % of
+// the leaf visits of the 1M-Gaussian 1080p view have more hits and take a
+// second pass).
+constexpr int kViewK = 8;
+
 // ---------------------------------------------------------------------------
 // Plain one-view kernel.
 // ---------------------------------------------------------------------------
@@ -64,7 +69,7 @@ static inline uint32_t interleave_grid(uint64_t n_logical) {
     return (uint32_t)(((n_logical + G * kXcds - 1) / (G * kXcds)) * (G * kXcds));
 }
 
-template <int K, bool DBG, int FMT, bool PROF = false>
+template <int K, bool DBG, int FMT, bool PROF = false, bool CONE = true>
 __global__ __launch_bounds__(64) void k_view(ViewArgs a) {
     // One wavefront per block = one 8x8-pixel tile, so a finished wave frees
     // its slot at once (per-ray cost varies widely across an image).  Groups
@@ -95,7 +100,7 @@ __global__ __launch_bounds__(64) void k_view(ViewArgs a) {
         dbg.tlast = clock64();
     }
     extern __shared__ uint2 lds_stack[];
-    trace_ray<K, DBG, PROF>(a.sc, o, d, a.depth, lds_stack + lane, rgb, A, &dbg);
+    trace_ray<K, DBG, PROF, CONE>(a.sc, o, d, a.depth, lds_stack + lane, rgb, A, &dbg);
     if (PROF) {
         for (int i = 0; i < 5; ++i) a.prof[8 * rid + i] = dbg.cyc[i];
         a.prof[8 * rid + 5] = dbg.pretests;
@@ -187,12 +192,49 @@ __global__ __launch_bounds__(256) void k_copy(float4* __restrict__ dst, const fl
     for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) dst[i] = src[i];
 }
 
+// Per-view cone records for the pre-test of rays leaving one origin e
+// (cone_pass).  For the sphere test's own f32 quantities a = c - e and
+// R = r + kappa*|a|_1 (same operations, same bits), sphere_pass accepting
+// implies, with eps = 2^-24 and exact arithmetic on the f32 inputs:
+//   |a x d| <= R' = R(1+4eps) + 10eps|a|   (rounding of the f32 cross product
+//                                           and of dist2 <= R*R),
+//   (a.d)^2 >= |a|^2 (1-16eps) - R'^2 =: Q^2  (|d|^2 >= 1-9eps for a
+//                                           normalised f32 direction),
+//   a.d > -R - 4eps|a|                      (t test with t_near >= 1e-2 > 0;
+//                                           f32 a.d is within 4eps|a|).
+// When Q >= R + 4eps|a| the second and third exclude a.d <= -Q, so
+// a.d >= Q and the f32 a.d >= q = Q - 4eps|a|.  q is formed in double and
+// rounded down; otherwise (origin near or inside the sphere, or r = inf) the
+// record accepts every direction.
+__global__ __launch_bounds__(256) void k_cone_prep(const float4* __restrict__ lsph, uint64_t n, float ex, float ey,
+                                                   float ez, float4* __restrict__ vcone) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+        const float4 sp = lsph[i];
+        const float ax = sp.x - ex, ay = sp.y - ey, az = sp.z - ez;
+        const uint32_t w = __float_as_uint(sp.w);
+        const float r = __half2float(__ushort_as_half((unsigned short)(w & 0xFFFFu)));
+        const float kap = __half2float(__ushort_as_half((unsigned short)(w >> 16)));
+        const float R = r + kap * (fabsf(ax) + fabsf(ay) + fabsf(az));
+        float q = -3.402823466e+38f;
+        if (R < 3.402823466e+38f) {
+            const double eps = 5.9604644775390625e-08;
+            const double a2 = (double)ax * ax + (double)ay * ay + (double)az * az;
+            const double na = sqrt(a2);
+            const double Rp = (double)R * (1.0 + 4.0 * eps) + 10.0 * eps * na;
+            const double Q2 = a2 * (1.0 - 16.0 * eps) - Rp * Rp;
+            const double lim = (double)R + 4.0 * eps * na;
+            if (Q2 > 0.0 && Q2 >= lim * lim) q = __double2float_rd(sqrt(Q2) - 4.0 * eps * na);
+        }
+        vcone[i] = make_float4(ax, ay, az, q);
+    }
+}
+
 __global__ void k_reset(uint32_t* c, int n) {
     if ((int)threadIdx.x < n) c[threadIdx.x] = 0;
 }
 
 template <int K>
-__global__ __launch_bounds__(256) void k_batch(BatchArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 4 : 1))) void k_batch(BatchArgs a) {
     const int lane = threadIdx.x & 63;
     extern __shared__ uint2 lds_stack[];
     uint2* const stk = lds_stack + (threadIdx.x >> 6) * (64 * a.depth) + lane;
@@ -218,33 +260,54 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs a) {
             if (a.items[mid].tile_begin <= tile) lo = mid;
             else hi = mid - 1;
         }
-        const WorkItem it = a.items[__builtin_amdgcn_readfirstlane(lo)];
+        // copy the work item word by word through readfirstlane so that every
+        // field (scene pointers, camera) is held in scalar registers
+        WorkItem it;
+        {
+            const uint32_t* src = reinterpret_cast<const uint32_t*>(a.items + __builtin_amdgcn_readfirstlane(lo));
+            uint32_t* dst = reinterpret_cast<uint32_t*>(&it);
+#pragma unroll
+            for (int w = 0; w < (int)(sizeof(WorkItem) / 4); ++w) dst[w] = __builtin_amdgcn_readfirstlane(src[w]);
+        }
         const uint32_t local = tile - it.tile_begin;
-        if (it.W > 0) {
-            const int tx = (int)(local % (uint32_t)it.tiles_x), ty = (int)(local / (uint32_t)it.tiles_x);
-            const int npx = a.tile_w * a.tile_h;
-            for (int p = lane; p < npx; p += 64) {
+        const bool is_view = it.W > 0;
+        const int tx = is_view ? (int)(local % (uint32_t)it.tiles_x) : 0;
+        const int ty = is_view ? (int)(local / (uint32_t)it.tiles_x) : 0;
+        const int npx = a.tile_w * a.tile_h;
+        for (int p0 = 0; p0 < npx; p0 += 64) {
+            const int p = p0 + lane;
+            float o[3], d[3];
+            bool live;
+            size_t dst;
+            if (is_view) {
                 // 8x8 sub-squares inside the tile keep each 64-lane step compact
                 const int sub = p >> 6, l = p & 63;
                 const int subs_x = a.tile_w >> 3;
                 const int x = tx * a.tile_w + (sub % subs_x) * 8 + (l & 7);
                 const int y = ty * a.tile_h + (sub / subs_x) * 8 + (l >> 3);
-                if (x >= it.W || y >= it.H) continue;
-                float o[3], d[3], rgb[3], A;
+                live = x < it.W && y < it.H;
                 make_ray(it.cam, x, y, o, d);
-                trace_ray<K, false>(it.sc, o, d, a.depth, stk, rgb, A, nullptr);
-                it.out[(size_t)(it.H - y - 1) * it.W + x] = make_float4(rgb[0], rgb[1], rgb[2], 1.0f - A);
-            }
-        } else {
-            const int npx = a.tile_w * a.tile_h;
-            for (int p = lane; p < npx; p += 64) {
+                dst = (size_t)(it.H - y - 1) * it.W + x; // framebuffer row flip
+            } else {
                 const uint64_t i = (uint64_t)local * npx + p;
-                if (i >= it.n_rays) continue;
-                const float o[3] = {it.ro[3 * i], it.ro[3 * i + 1], it.ro[3 * i + 2]};
-                const float d[3] = {it.rd[3 * i], it.rd[3 * i + 1], it.rd[3 * i + 2]};
+                live = i < it.n_rays;
+                o[0] = o[1] = o[2] = 0.f;
+                d[0] = d[1] = 0.f;
+                d[2] = 1.f;
+                if (live) {
+                    o[0] = it.ro[3 * i];
+                    o[1] = it.ro[3 * i + 1];
+                    o[2] = it.ro[3 * i + 2];
+                    d[0] = it.rd[3 * i];
+                    d[1] = it.rd[3 * i + 1];
+                    d[2] = it.rd[3 * i + 2];
+                }
+                dst = i;
+            }
+            if (live) {
                 float rgb[3], A;
                 trace_ray<K, false>(it.sc, o, d, a.depth, stk, rgb, A, nullptr);
-                it.out[i] = make_float4(rgb[0], rgb[1], rgb[2], 1.0f - A);
+                it.out[dst] = make_float4(rgb[0], rgb[1], rgb[2], 1.0f - A);
             }
         }
     }
@@ -284,6 +347,14 @@ struct kdgrt_scene {
     void* d_shade = nullptr;
     void* d_sph = nullptr;
     void* d_lsph = nullptr;
+    // Cone records of the plain view launches, one table per scene: launches
+    // on one stream run in order; a launch on another stream first waits for
+    // the last launch that used the table (cone_ev).
+    void* d_vcone = nullptr;
+    hipEvent_t cone_ev = nullptr;
+    hipStream_t cone_stream = nullptr;
+    bool cone_used = false;
+    std::mutex cone_mu;
     uint64_t bytes = 0;
     uint32_t n_nodes = 0;
     uint64_t n_offsets = 0, n_gaussians = 0;
@@ -574,7 +645,7 @@ kdgrt_status kdgrt_copy_f32x4(kdgrt_context* ctx, void* dst, const void* src, ui
 uint64_t kdgrt_scene_bytes_estimate(uint32_t n_nodes, uint64_t n_offsets, uint64_t n_gaussians) {
     auto r = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
     return r(8ull * n_nodes) + r(4ull * n_offsets) + r(80ull * n_gaussians) + r(192ull * n_gaussians) +
-           r(16ull * n_gaussians) + r(16ull * n_offsets);
+           r(16ull * n_gaussians) + 2 * r(16ull * (n_offsets + kLsphPad));
 }
 
 kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uint32_t n_nodes,
@@ -588,7 +659,10 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
     HIP_TRY(hipSetDevice(ctx->device));
     // Repack: 48-B geometry records (pos|opacity, inverse scale|k_scale,
     // quaternion) and 192-B SH records.
-    std::vector<float4> geom(5 * n_g), shade(12 * n_g), sph(n_g), lsph(std::max<uint64_t>(1, n_offsets));
+    std::vector<float4> geom(5 * n_g), shade(12 * n_g), sph(n_g);
+    // leaf-slot spheres plus kLsphPad zero records: the pre-test reads whole
+    // groups of four past a leaf's end and masks the extra bits off
+    std::vector<float4> lsph(n_offsets + kLsphPad, make_float4(0.f, 0.f, 0.f, 0.f));
     for (uint64_t i = 0; i < n_g; ++i) {
         const kdgrt_gaussian& s = g[i];
         float inv[3], kinv;
@@ -636,6 +710,8 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
         if (sc->d_shade) hipFree(sc->d_shade);
         if (sc->d_sph) hipFree(sc->d_sph);
         if (sc->d_lsph) hipFree(sc->d_lsph);
+        if (sc->d_vcone) hipFree(sc->d_vcone);
+        if (sc->cone_ev) hipEventDestroy(sc->cone_ev);
         delete sc;
         return set_error(e == hipErrorOutOfMemory ? KDGRT_ERR_OUT_OF_MEMORY : KDGRT_ERR_DEVICE,
                          std::string("scene_create: ") + what + ": " + hipGetErrorString(e));
@@ -647,6 +723,8 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
     if ((e = hipMalloc(&sc->d_shade, 192ull * n_g)) != hipSuccess) return fail(e, "sh");
     if ((e = hipMalloc(&sc->d_sph, 16ull * n_g)) != hipSuccess) return fail(e, "spheres");
     if ((e = hipMalloc(&sc->d_lsph, 16ull * lsph.size())) != hipSuccess) return fail(e, "leaf spheres");
+    if ((e = hipMalloc(&sc->d_vcone, 16ull * lsph.size())) != hipSuccess) return fail(e, "view cones");
+    if ((e = hipEventCreateWithFlags(&sc->cone_ev, hipEventDisableTiming)) != hipSuccess) return fail(e, "event");
     if ((e = hipMemcpy(sc->d_lsph, lsph.data(), 16ull * lsph.size(), hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "copy");
     if ((e = hipMemcpy(sc->d_sph, sph.data(), 16ull * n_g, hipMemcpyHostToDevice)) != hipSuccess)
@@ -691,6 +769,8 @@ kdgrt_status kdgrt_scene_destroy(kdgrt_scene* sc) {
     hipFree(sc->d_shade);
     hipFree(sc->d_sph);
     hipFree(sc->d_lsph);
+    hipFree(sc->d_vcone);
+    hipEventDestroy(sc->cone_ev);
     delete sc;
     return KDGRT_OK;
 }
@@ -727,6 +807,28 @@ static void fill_view_args(ViewArgs& a, const kdgrt_scene* sc, const kdgrt_camer
     a.seq_cap = 0;
 }
 
+// Launches one plain view: the cone records of the view's origin into the
+// scene's table, then the view kernel, both on `s`.  Launches of one scene on
+// different streams are ordered through the table's event.
+typedef void (*ViewKernel)(ViewArgs);
+static kdgrt_status launch_view(const kdgrt_scene* csc, ViewArgs& a, ViewKernel kern, hipStream_t s) {
+    kdgrt_scene* sc = const_cast<kdgrt_scene*>(csc);
+    std::lock_guard<std::mutex> lock(sc->cone_mu);
+    if (sc->cone_used && sc->cone_stream != s) HIP_TRY(hipStreamWaitEvent(s, sc->cone_ev, 0));
+    const uint64_t n = sc->n_offsets + kLsphPad;
+    const uint32_t prep_grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_cone_prep, dim3(prep_grid), dim3(256), 0, s, (const float4*)sc->d_lsph, n, a.cam.eye[0],
+                       a.cam.eye[1], a.cam.eye[2], (float4*)sc->d_vcone);
+    a.sc.vcone = (const float4*)sc->d_vcone;
+    const dim3 grid(interleave_grid<64>((uint64_t)a.blocks_x * a.blocks_y * 64)), block(64);
+    hipLaunchKernelGGL(kern, grid, block, stack_lds_bytes(a.depth, 64), s, a);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(sc->cone_ev, s));
+    sc->cone_stream = s;
+    sc->cone_used = true;
+    return KDGRT_OK;
+}
+
 kdgrt_status kdgrt_render(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgrt_camera_info* cam, int W, int H,
                           int depth, int fmt, void* out, void* stream) {
     kdgrt_status st = check_view(ctx, sc, cam, W, H, depth, out);
@@ -735,13 +837,10 @@ kdgrt_status kdgrt_render(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgrt
         return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render: unknown output format");
     ViewArgs a;
     fill_view_args(a, sc, cam, W, H, depth, out);
-    const dim3 grid(interleave_grid<64>((uint64_t)a.blocks_x * a.blocks_y * 64)), block(64);
-    hipStream_t s = (hipStream_t)stream;
-    const size_t lds = stack_lds_bytes(depth, 64);
-    if (fmt == KDGRT_OUT_RGBT_F32) hipLaunchKernelGGL((k_view<8, false, KDGRT_OUT_RGBT_F32>), grid, block, lds, s, a);
-    else hipLaunchKernelGGL((k_view<8, false, KDGRT_OUT_RGB_F32>), grid, block, lds, s, a);
-    HIP_TRY(hipGetLastError());
-    return KDGRT_OK;
+    return launch_view(sc, a,
+                       fmt == KDGRT_OUT_RGBT_F32 ? k_view<kViewK, false, KDGRT_OUT_RGBT_F32>
+                                                 : k_view<kViewK, false, KDGRT_OUT_RGB_F32>,
+                       (hipStream_t)stream);
 }
 
 kdgrt_status kdgrt_render_debug(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgrt_camera_info* cam, int W,
@@ -756,11 +855,7 @@ kdgrt_status kdgrt_render_debug(kdgrt_context* ctx, const kdgrt_scene* sc, const
     a.hashes = hashes;
     a.seq = seq;
     a.seq_cap = seq ? seq_cap : 0;
-    const dim3 grid(interleave_grid<64>((uint64_t)a.blocks_x * a.blocks_y * 64)), block(64);
-    hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL((k_view<8, true, KDGRT_OUT_RGBT_F32>), grid, block, stack_lds_bytes(depth, 64), s, a);
-    HIP_TRY(hipGetLastError());
-    return KDGRT_OK;
+    return launch_view(sc, a, k_view<kViewK, true, KDGRT_OUT_RGBT_F32>, (hipStream_t)stream);
 }
 
 kdgrt_status kdgrt_render_profile(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgrt_camera_info* cam, int W,
@@ -771,11 +866,7 @@ kdgrt_status kdgrt_render_profile(kdgrt_context* ctx, const kdgrt_scene* sc, con
     ViewArgs a;
     fill_view_args(a, sc, cam, W, H, 12, out);
     a.prof = phase_cycles; // 8 x u64 per ray
-    const dim3 grid(interleave_grid<64>((uint64_t)a.blocks_x * a.blocks_y * 64)), block(64);
-    hipLaunchKernelGGL((k_view<8, false, KDGRT_OUT_RGBT_F32, true>), grid, block, stack_lds_bytes(12, 64),
-                       (hipStream_t)stream, a);
-    HIP_TRY(hipGetLastError());
-    return KDGRT_OK;
+    return launch_view(sc, a, k_view<kViewK, false, KDGRT_OUT_RGBT_F32, true>, (hipStream_t)stream);
 }
 
 static kdgrt_status launch_rays(kdgrt_context* ctx, const kdgrt_scene* sc, const float* ro, const float* rd,
@@ -789,8 +880,8 @@ static kdgrt_status launch_rays(kdgrt_context* ctx, const kdgrt_scene* sc, const
     const dim3 grid(interleave_grid<64>((n + 63) / 64)), block(64);
     hipStream_t s = (hipStream_t)stream;
     const size_t lds = stack_lds_bytes(depth, 64);
-    if (dbg) hipLaunchKernelGGL((k_rays<8, true>), grid, block, lds, s, a);
-    else hipLaunchKernelGGL((k_rays<8, false>), grid, block, lds, s, a);
+    if (dbg) hipLaunchKernelGGL((k_rays<kViewK, true>), grid, block, lds, s, a);
+    else hipLaunchKernelGGL((k_rays<kViewK, false>), grid, block, lds, s, a);
     HIP_TRY(hipGetLastError());
     return KDGRT_OK;
 }
@@ -835,7 +926,7 @@ kdgrt_status kdgrt_tuning_default(kdgrt_context* ctx, kdgrt_tuning* t) {
     t->tile_w = 8;
     t->tile_h = 8;
     t->stack_depth = 12;
-    t->hit_capacity = 8;
+    t->hit_capacity = kViewK;
     t->waves_per_cu = 32;
     t->block_threads = 64;
     return KDGRT_OK;

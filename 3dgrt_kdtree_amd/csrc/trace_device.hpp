@@ -11,7 +11,8 @@
 //   sortHits (stable by t)    :415-426
 //   composite loop            :1608-1688, particleResponse<4> :681-685,
 //                              eval_sh_final(3) :487-558
-// with the documented substitutions __fdividef -> '/', rsqrtf -> 1/sqrtf,
+// with the documented substitutions __fdividef(a, b) -> a * (1/b) (IEEE
+// reciprocal of the direction component, once per ray), rsqrtf -> 1/sqrtf,
 // expf -> kd_expf, no FMA contraction (-ffp-contract=off).
 //
 // Machine plan (not the reference's):
@@ -50,6 +51,7 @@ struct DevScene {
     const float4* __restrict__ shade; // 12 x float4 per Gaussian, channel-major (f_dc[c], f_rest[15c..15c+14])
     const float4* __restrict__ sph;   // 1 x float4 per Gaussian: centre, packed (radius, kappa) as 2 x f16
     const float4* __restrict__ lsph;  // the same records in leaf-slot order: lsph[k] = sph[offs[k]]
+    const float4* __restrict__ vcone; // per-view cone records in leaf-slot order (views only, else null)
     float bmin[3];
     float bmax[3];
 };
@@ -85,6 +87,10 @@ struct RayDbg {
 // A validated tree visits each leaf at most once per ray; this cap is a
 // guard against malformed input only and is never reached otherwise.
 constexpr int kLeafGuard = 1 << 22;
+
+// Zero records after the last leaf-slot sphere: the pre-test reads groups of
+// four slots and may run up to three past the end of the last leaf.
+constexpr uint64_t kLsphPad = 4;
 
 __device__ __forceinline__ void make_ray(const CamInfo& c, int x, int y, float o[3], float d[3]) {
     const float sx = (float)x + 0.5f, sy = (float)y + 0.5f;
@@ -177,6 +183,16 @@ __device__ __forceinline__ bool sphere_pass(const float o[3], const float d[3], 
     const float cx = ay * d[2] - az * d[1], cy = az * d[0] - ax * d[2], cz = ax * d[1] - ay * d[0];
     const float dist2 = cx * cx + cy * cy + cz * cz;
     return (dist2 <= R * R) & (tp + R >= t_near) & (tp - R <= t_far);
+}
+
+// Pre-test for rays that share the origin o of a view.  The record holds
+// a = centre - o (the f32 difference the sphere test forms) and a threshold
+// q: every candidate that sphere_pass accepts has a.d >= q, so testing
+// a.d >= q instead keeps all of them (q is derived and rounded down on the
+// device by k_cone_prep).  a.d is formed in sphere_pass's order.
+__device__ __forceinline__ bool cone_pass(const float d[3], const float4 c) {
+    const float tp = c.x * d[0] + c.y * d[1] + c.z * d[2];
+    return tp >= c.w;
 }
 
 struct TestOut {
@@ -279,189 +295,271 @@ __device__ __forceinline__ void insert_hit(float (&kt)[K], float (&kg)[K], float
     n = n + 1 < K ? n + 1 : K;
 }
 
-// stk: this lane's short-stack ring in LDS, entry e at stk[64 * e]
-// (entry-major over the 64 lanes of the wave), `depth` entries.
-template <int K, bool DBG, bool PROF = false>
-__device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], const float d[3], int depth,
-                                          uint2* __restrict__ stk, float rgb[3], float& A, RayDbg* dbg) {
-    rgb[0] = rgb[1] = rgb[2] = 0.0f;
-    A = 0.0f;
+// Value select by split axis.  The operands are values, not lvalues: a
+// conditional over adjacent struct fields becomes an indexed load and moves
+// the whole ray state to scratch memory.
+__device__ __forceinline__ float sel_axis(uint32_t axis, float x, float y, float z) {
+    return axis == 0 ? x : (axis == 1 ? y : z);
+}
+
+// State of one ray between leaf steps (registers of its lane).
+struct RayState {
+    float ox, oy, oz, dx, dy, dz;
+    float ix, iy, iz; // 1 / direction
+    uint32_t signs;   // bit k = sign bit of direction component k
+    float t_near, t_far, t_scene_far;
+    uint2 node;
+    uint32_t top, quant; // ring stack index and fill (cudaRenderer.cu:189-210)
+    float rgb[3];
+    float A;
+    int guard;
+};
+
+// Starts a ray: scene-box slab test (cudaRenderer.cu:379-399, 1536-1537),
+// root node, empty ring.  Returns false when the ray misses the box; its
+// result is then black with T = 1.
+__device__ __forceinline__ bool ray_begin(RayState& st, const DevScene& sc, const float o[3], const float d[3],
+                                          int depth) {
+    st.rgb[0] = st.rgb[1] = st.rgb[2] = 0.0f;
+    st.A = 0.0f;
+    st.ox = o[0];
+    st.oy = o[1];
+    st.oz = o[2];
+    st.dx = d[0];
+    st.dy = d[1];
+    st.dz = d[2];
+    const float inv[3] = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
+    st.ix = inv[0];
+    st.iy = inv[1];
+    st.iz = inv[2];
+    st.signs = (__float_as_uint(d[0]) >> 31) | ((__float_as_uint(d[1]) >> 31) << 1) |
+               ((__float_as_uint(d[2]) >> 31) << 2);
     float t_near = 1e-2f, t_far = 3.402823466e+38f;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const float l1 = (sc.bmin[k] - o[k]) / d[k];
-        const float l2 = (sc.bmax[k] - o[k]) / d[k];
+        const float l1 = (sc.bmin[k] - o[k]) * inv[k];
+        const float l2 = (sc.bmax[k] - o[k]) * inv[k];
         t_near = fmaxf(fminf(l1, l2), t_near);
         t_far = fminf(fmaxf(l1, l2), t_far);
     }
-    if (!((t_far >= t_near) & (t_far >= 0.f))) return;
-    const float t_scene_far = t_far;
+    st.t_near = t_near;
+    st.t_far = t_far;
+    st.t_scene_far = t_far;
+    st.top = (uint32_t)depth - 1u;
+    st.quant = 0;
+    st.guard = 0;
+    if (!((t_far >= t_near) & (t_far >= 0.f))) return false;
+    st.node = sc.nodes[0];
+    return true;
+}
 
-    // ring stack state (cudaRenderer.cu:189-210): _top starts at depth-1
-    uint32_t top = (uint32_t)depth - 1u, quant = 0;
-    // Scalar copies for the per-node axis select: selecting between array
-    // elements lets the compiler turn the select into an indexed load and
-    // put the ray in scratch memory.
-    const float ox = o[0], oy = o[1], oz = o[2], dx = d[0], dy = d[1], dz = d[2];
-
-    uint2 node = sc.nodes[0];
-    int guard = 0;
-    while (A < 0.99f) {
-        while ((node.x & 7u) != 3u) {
-            const uint32_t child = node.x >> 3;
-            const uint32_t axis = node.x & 3u;
-            const float oa = axis == 0 ? ox : (axis == 1 ? oy : oz);
-            const float da = axis == 0 ? dx : (axis == 1 ? dy : dz);
-            const float t_split = (__uint_as_float(node.y) - oa) / da;
-            const uint32_t sign = __float_as_uint(da) >> 31;
-            const uint32_t idx = child + (sign ^ (t_split <= t_near ? 1u : 0u));
-            if (t_near < t_split && t_split < t_far) {
-                if (++top == (uint32_t)depth) top = 0; // push overwrites the oldest when full
-                quant = quant + 1 < (uint32_t)depth ? quant + 1 : (uint32_t)depth;
-                stk[64 * top] = make_uint2(child + (sign ^ 1u), __float_as_uint(t_far));
-                t_far = t_split;
-            }
-            if (DBG) dbg->cnt[0]++;
-            node = sc.nodes[idx];
+// Descent from st.node to the next leaf (cudaRenderer.cu:1555-1573) with
+// the ring short stack (:189-210).  stk is this lane's ring in LDS, entry e
+// at stk[64 * e] (entry-major over the wave's 64 lanes).
+template <bool DBG>
+__device__ __forceinline__ void descend(RayState& st, const DevScene& sc, int depth, uint2* __restrict__ stk,
+                                        RayDbg* dbg) {
+    while ((st.node.x & 7u) != 3u) {
+        const uint32_t child = st.node.x >> 3;
+        const uint32_t axis = st.node.x & 3u;
+        // axis 3 cannot occur in an inner node (3 tags a leaf); as in the
+        // reference's get_dir_pos, anything but 0 and 1 selects z
+        const float oa = sel_axis(axis, st.ox, st.oy, st.oz);
+        const float ia = sel_axis(axis, st.ix, st.iy, st.iz);
+        const float t_split = (__uint_as_float(st.node.y) - oa) * ia;
+        const uint32_t sign = (st.signs >> (axis > 2u ? 2u : axis)) & 1u;
+        const uint32_t idx = child + (sign ^ (t_split <= st.t_near ? 1u : 0u));
+        if (st.t_near < t_split && t_split < st.t_far) {
+            if (++st.top == (uint32_t)depth) st.top = 0; // push overwrites the oldest when full
+            st.quant = st.quant + 1 < (uint32_t)depth ? st.quant + 1 : (uint32_t)depth;
+            stk[64 * st.top] = make_uint2(child + (sign ^ 1u), __float_as_uint(st.t_far));
+            st.t_far = t_split;
         }
-        KD_PROF_MARK(0);
-
-        // ---- leaf ----
-        const uint32_t base = node.y, count = node.x >> 3;
-        if (DBG) {
-            dbg->cnt[1]++;
-            dbg->cnt[2] += (int32_t)count;
-        }
-        bool have_lb = false, done = false;
-        float lb_t = 0.f;
-        uint32_t lb_s = 0;
-        int first_hits = 0;
-        while (true) {
-            float kt[K], kg[K], ko[K];
-            uint32_t kid[K];
-#pragma unroll
-            for (int i = 0; i < K; ++i) {
-                kt[i] = 0.f;
-                kg[i] = 0.f;
-                ko[i] = 0.f;
-                kid[i] = 0;
-            }
-            int n = 0;
-            bool overflow = false;
-            // Windows of 64 slots: the cheap pre-test runs for every slot and
-            // builds a survivor mask; the exact test then runs only for
-            // survivors, in slot order, so a wavefront executes the exact test
-            // as often as its busiest lane needs it rather than once for every
-            // slot some lane still needs.
-            for (uint32_t w0 = 0; w0 < count; w0 += 64) {
-                const uint32_t wn = (count - w0) < 64u ? (count - w0) : 64u;
-                uint64_t mask = 0;
-                for (uint32_t j0 = 0; j0 < wn; j0 += 4) {
-                    // four leaf-ordered bound spheres per step, one contiguous read
-                    float4 sp[4];
-#pragma unroll
-                    for (int j = 0; j < 4; ++j)
-                        sp[j] = (j0 + j < wn) ? sc.lsph[base + w0 + j0 + j] : make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        if (j0 + j < wn && sphere_pass(o, d, sp[j], t_near, t_far)) mask |= 1ull << (j0 + j);
-                        if (PROF && j0 + j < wn) dbg->pretests++;
-                    }
-                }
-                KD_PROF_MARK(1);
-                if (PROF) dbg->survivors += (uint32_t)__builtin_popcountll(mask);
-                // exact tests, two survivors per step so their loads overlap
-                while (mask) {
-                    const uint32_t j1 = (uint32_t)__builtin_ctzll(mask);
-                    mask &= mask - 1;
-                    const bool two = mask != 0;
-                    const uint32_t j2 = two ? (uint32_t)__builtin_ctzll(mask) : j1;
-                    if (two) mask &= mask - 1;
-                    const uint32_t s1 = w0 + j1, s2 = w0 + j2;
-                    const uint32_t g1 = sc.offs[base + s1];
-                    const uint32_t g2 = two ? sc.offs[base + s2] : g1;
-                    const float4* r1p = sc.geom + 5 * (size_t)g1;
-                    const float4* r2p = sc.geom + 5 * (size_t)g2;
-                    const float4 a1 = r1p[0], b1 = r1p[1], c1 = r1p[2], e1 = r1p[3], f1 = r1p[4];
-                    const float4 a2 = r2p[0], b2 = r2p[1], c2 = r2p[2], e2 = r2p[3], f2 = r2p[4];
-#pragma unroll
-                    for (int u = 0; u < 2; ++u) {
-                        if (u == 1 && !two) break;
-                        const uint32_t s = u ? s2 : s1, gid = u ? g2 : g1;
-                        const TestOut r = u ? ray_gaussian(o, d, a2, b2, c2, e2, f2, t_near, t_far)
-                                            : ray_gaussian(o, d, a1, b1, c1, e1, f1, t_near, t_far);
-                        if (!r.hit) continue;
-                        if (have_lb && !(r.t > lb_t || (r.t == lb_t && s > lb_s))) continue;
-                        if (DBG && !have_lb) first_hits++;
-                        insert_hit<K>(kt, kg, ko, kid, n, overflow, r, gid);
-                    }
-                }
-                KD_PROF_MARK(2);
-            }
-            if (DBG && !have_lb) {
-                dbg->cnt[3] += first_hits;
-                if (first_hits > dbg->cnt[5]) dbg->cnt[5] = first_hits;
-            }
-#pragma unroll
-            for (int i = 0; i < K; ++i) {
-                if (i >= n || done) break;
-                if (DBG) dbg->cnt[6]++;
-                // particleResponse<4> and the skip rule (cudaRenderer.cu:1655-1671)
-                const float gd = kg[i];
-                const float gres = kd_expf(-0.0555555555556f * gd * gd);
-                const float alpha = fminf(0.99f, ko[i] * gres);
-                if (gres < 0.0113f || alpha < 1.0f / 255.0f) continue;
-                const uint32_t gid = kid[i];
-                // The basis depends only on the ray direction; it is formed here,
-                // per blended sample, rather than held in 16 registers per ray.
-                float bx = dx, by = dy, bz = dz;
-                asm volatile("" : "+v"(bx), "+v"(by), "+v"(bz));
-                const float dv[3] = {bx, by, bz};
-                float shs[16];
-                sh_basis(dv, shs);
-                float c[3];
-                sh_radiance(sc.shade + 12 * (size_t)gid, shs, c);
-                const float w = 1.0f - A;
-                rgb[0] = rgb[0] + (c[0] * alpha) * w;
-                rgb[1] = rgb[1] + (c[1] * alpha) * w;
-                rgb[2] = rgb[2] + (c[2] * alpha) * w;
-                A = A + alpha * (1.0f - A);
-                if (DBG) {
-                    dbg->cnt[4]++;
-                    dbg->hash = (dbg->hash ^ (uint64_t)gid) * 0x100000001b3ull;
-                    if (dbg->seq && dbg->nseq < dbg->seq_cap) dbg->seq[dbg->nseq] = gid;
-                    dbg->nseq++;
-                }
-                if (A > 0.99f) done = true;
-            }
-            KD_PROF_MARK(3);
-            if (done || !overflow) break;
-            // Re-scan above the last composited key (t, slot).  Leaf ID lists
-            // hold each ID once (validated), so the slot is the ID's position.
-            lb_t = kt[K - 1];
-            lb_s = 0;
-            while (lb_s < count && sc.offs[base + lb_s] != kid[K - 1]) ++lb_s;
-            have_lb = true;
-        }
-
-        if ((A > 0.99f) | (t_far >= t_scene_far)) break;
-        if (quant == 0) {
-            node = sc.nodes[0];
-            t_near = t_far;
-            t_far = t_scene_far;
-            if (DBG) dbg->cnt[7]++;
-        } else {
-            const uint2 e = stk[64 * top];
-            if (top == 0) top = (uint32_t)depth;
-            --top;
-            --quant;
-            node = sc.nodes[e.x];
-            t_near = t_far;
-            t_far = __uint_as_float(e.y);
-        }
-        if (++guard > kLeafGuard) break;
-        KD_PROF_MARK(4);
+        if (DBG) dbg->cnt[0]++;
+        st.node = sc.nodes[idx];
     }
+}
+
+// After a leaf (cudaRenderer.cu:1690-1708): termination test, then pop or
+// restart at the root.  Returns true while the ray continues.
+template <bool DBG>
+__device__ __forceinline__ bool next_leaf(RayState& st, const DevScene& sc, int depth, const uint2* __restrict__ stk,
+                                          RayDbg* dbg) {
+    if ((st.A > 0.99f) | (st.t_far >= st.t_scene_far)) return false;
+    if (st.quant == 0) {
+        st.node = sc.nodes[0]; // restart from the root
+        st.t_near = st.t_far;
+        st.t_far = st.t_scene_far;
+        if (DBG) dbg->cnt[7]++;
+    } else {
+        const uint2 e = stk[64 * st.top];
+        if (st.top == 0) st.top = (uint32_t)depth;
+        --st.top;
+        --st.quant;
+        st.node = sc.nodes[e.x];
+        st.t_near = st.t_far;
+        st.t_far = __uint_as_float(e.y);
+    }
+    return (++st.guard <= kLeafGuard) && (st.A < 0.99f);
+}
+
+// The leaf in st.node (cudaRenderer.cu:1576-1688): the leaf's candidates in
+// stored order, exact test, stable order by t, composite with the skip rule
+// and early termination.  The K-entry register buffer holds the leaf's hits
+// by (t, slot); a leaf with more than K hits is finished by further passes
+// that keep only keys above the last composited one.
+template <int K, bool DBG, bool PROF, bool CONE>
+__device__ __forceinline__ void leaf(RayState& st, const DevScene& sc, RayDbg* dbg) {
+    const uint32_t base = st.node.y, count = st.node.x >> 3;
+    if (DBG) {
+        dbg->cnt[1]++;
+        dbg->cnt[2] += (int32_t)count;
+    }
+    const float o[3] = {st.ox, st.oy, st.oz}, d[3] = {st.dx, st.dy, st.dz};
+    bool have_lb = false, done = false;
+    float lb_t = 0.f;
+    uint32_t lb_s = 0;
+    int first_hits = 0;
+    while (true) {
+        float kt[K], kg[K], ko[K];
+        uint32_t kid[K];
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            kt[i] = 0.f;
+            kg[i] = 0.f;
+            ko[i] = 0.f;
+            kid[i] = 0;
+        }
+        int n = 0;
+        bool overflow = false;
+        // Windows of 64 slots: the cheap pre-test runs for every slot and
+        // builds a survivor mask; the exact test then runs only for
+        // survivors, in slot order, so a wavefront executes the exact test as
+        // often as its busiest lane needs it rather than once for every slot
+        // some lane still needs.
+        for (uint32_t w0 = 0; w0 < count; w0 += 64) {
+            const uint32_t wn = (count - w0) < 64u ? (count - w0) : 64u;
+            uint64_t mask = 0;
+            // four slots per step, one contiguous read; the group may run past
+            // the leaf (the record arrays are padded) and bits at or above wn
+            // are cleared afterwards
+            for (uint32_t j0 = 0; j0 < wn; j0 += 4) {
+                uint32_t bits;
+                if (CONE) {
+                    const float4* cp = sc.vcone + base + w0 + j0;
+                    const float4 c0 = cp[0], c1 = cp[1], c2 = cp[2], c3 = cp[3];
+                    bits = (cone_pass(d, c0) ? 1u : 0u) | (cone_pass(d, c1) ? 2u : 0u) |
+                           (cone_pass(d, c2) ? 4u : 0u) | (cone_pass(d, c3) ? 8u : 0u);
+                } else {
+                    const float4* sp = sc.lsph + base + w0 + j0;
+                    const float4 s0 = sp[0], s1 = sp[1], s2 = sp[2], s3 = sp[3];
+                    bits = (sphere_pass(o, d, s0, st.t_near, st.t_far) ? 1u : 0u) |
+                           (sphere_pass(o, d, s1, st.t_near, st.t_far) ? 2u : 0u) |
+                           (sphere_pass(o, d, s2, st.t_near, st.t_far) ? 4u : 0u) |
+                           (sphere_pass(o, d, s3, st.t_near, st.t_far) ? 8u : 0u);
+                }
+                mask |= (uint64_t)bits << j0;
+            }
+            if (wn < 64) mask &= (1ull << wn) - 1ull;
+            if (PROF) dbg->pretests += wn;
+            KD_PROF_MARK(1);
+            if (PROF) dbg->survivors += (uint32_t)__popcll(mask);
+            // exact tests, two survivors per step so that their loads overlap
+            while (mask) {
+                const uint32_t j1 = (uint32_t)__builtin_ctzll(mask);
+                mask &= mask - 1;
+                const bool two = mask != 0;
+                const uint32_t j2 = two ? (uint32_t)__builtin_ctzll(mask) : j1;
+                if (two) mask &= mask - 1;
+                const uint32_t s1 = w0 + j1, s2 = w0 + j2;
+                const uint32_t g1 = sc.offs[base + s1];
+                const uint32_t g2 = two ? sc.offs[base + s2] : g1;
+                const float4* r1p = sc.geom + 5 * (size_t)g1;
+                const float4* r2p = sc.geom + 5 * (size_t)g2;
+                const float4 a1 = r1p[0], b1 = r1p[1], c1 = r1p[2], e1 = r1p[3], f1 = r1p[4];
+                const float4 a2 = r2p[0], b2 = r2p[1], c2 = r2p[2], e2 = r2p[3], f2 = r2p[4];
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    if (u == 1 && !two) break;
+                    const uint32_t sl = u ? s2 : s1, gid = u ? g2 : g1;
+                    const TestOut r = u ? ray_gaussian(o, d, a2, b2, c2, e2, f2, st.t_near, st.t_far)
+                                        : ray_gaussian(o, d, a1, b1, c1, e1, f1, st.t_near, st.t_far);
+                    if (!r.hit) continue;
+                    if (have_lb && !(r.t > lb_t || (r.t == lb_t && sl > lb_s))) continue;
+                    if (DBG && !have_lb) first_hits++;
+                    insert_hit<K>(kt, kg, ko, kid, n, overflow, r, gid);
+                }
+            }
+            KD_PROF_MARK(2);
+        }
+        if (DBG && !have_lb) {
+            dbg->cnt[3] += first_hits;
+            if (first_hits > dbg->cnt[5]) dbg->cnt[5] = first_hits;
+        }
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            if (i >= n || done) break;
+            if (DBG) dbg->cnt[6]++;
+            // particleResponse<4> and the skip rule (cudaRenderer.cu:1655-1671)
+            const float gd = kg[i];
+            const float gres = kd_expf(-0.0555555555556f * gd * gd);
+            const float alpha = fminf(0.99f, ko[i] * gres);
+            if (gres < 0.0113f || alpha < 1.0f / 255.0f) continue;
+            const uint32_t gid = kid[i];
+            // The basis depends only on the ray direction; it is formed here,
+            // per blended sample, rather than held in 16 registers per ray.
+            float bx = st.dx, by = st.dy, bz = st.dz;
+            asm volatile("" : "+v"(bx), "+v"(by), "+v"(bz));
+            const float dv[3] = {bx, by, bz};
+            float shs[16];
+            sh_basis(dv, shs);
+            float c[3];
+            sh_radiance(sc.shade + 12 * (size_t)gid, shs, c);
+            const float w = 1.0f - st.A;
+            st.rgb[0] = st.rgb[0] + (c[0] * alpha) * w;
+            st.rgb[1] = st.rgb[1] + (c[1] * alpha) * w;
+            st.rgb[2] = st.rgb[2] + (c[2] * alpha) * w;
+            st.A = st.A + alpha * (1.0f - st.A);
+            if (DBG) {
+                dbg->cnt[4]++;
+                dbg->hash = (dbg->hash ^ (uint64_t)gid) * 0x100000001b3ull;
+                if (dbg->seq && dbg->nseq < dbg->seq_cap) dbg->seq[dbg->nseq] = gid;
+                dbg->nseq++;
+            }
+            if (st.A > 0.99f) done = true;
+        }
+        KD_PROF_MARK(3);
+        if (done || !overflow) break;
+        // Next pass: keys above the last composited (t, slot).  Leaf ID lists
+        // hold each ID once (validated), so the slot is the ID's position.
+        lb_t = kt[K - 1];
+        lb_s = 0;
+        while (lb_s < count && sc.offs[base + lb_s] != kid[K - 1]) ++lb_s;
+        have_lb = true;
+    }
+}
+
+// Whole ray: outer loop of cudaRenderer.cu:1552-1709.  CONE: the ray starts
+// at the view origin that sc.vcone was prepared for.  stk: this lane's ring
+// (entry e at stk[64 * e]).
+template <int K, bool DBG, bool PROF = false, bool CONE = false>
+__device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], const float d[3], int depth,
+                                          uint2* __restrict__ stk, float rgb[3], float& A, RayDbg* dbg) {
+    RayState st;
+    if (ray_begin(st, sc, o, d, depth)) {
+        while (true) {
+            descend<DBG>(st, sc, depth, stk, dbg);
+            KD_PROF_MARK(0);
+            leaf<K, DBG, PROF, CONE>(st, sc, dbg);
+            const bool more = next_leaf<DBG>(st, sc, depth, stk, dbg);
+            KD_PROF_MARK(4);
+            if (!more) break;
+        }
+    }
+    rgb[0] = st.rgb[0];
+    rgb[1] = st.rgb[1];
+    rgb[2] = st.rgb[2];
+    A = st.A;
 }
 
 } // namespace kdgrt_dev

@@ -15,7 +15,10 @@
 // Substitutions for CUDA intrinsics whose bits cannot be reproduced off
 // NVIDIA hardware (the reference is built without fast-math, so '/', sqrtf
 // are IEEE there too):
-//   __fdividef(a,b) -> a / b         (cudaRenderer.cu:383-394, 1560)
+//   __fdividef(a,b) -> a * (1/b)     (cudaRenderer.cu:383-394, 1560): the
+//                      intrinsic is a multiply by an approximate reciprocal
+//                      (PTX div.approx.f32); here 1/b is the IEEE reciprocal
+//                      of the ray direction component, formed once per ray
 //   rsqrtf(x)       -> 1.0f/sqrtf(x) (cudaRenderer.cu:55)
 //   expf            -> kd_expf       (own Cody-Waite exp, shared with the kernel)
 //   nvcc FMA contraction -> none    (built with -ffp-contract=off)
@@ -206,8 +209,9 @@ void trace(const Scene& sc, V3<T> o, V3<T> d, int D, V3<T>& color, T& A, RayOut&
     {
         const T po[3] = {o.x, o.y, o.z}, pd[3] = {d.x, d.y, d.z};
         for (int k = 0; k < 3; ++k) {
-            T l1 = (T(sc.bmin[k]) - po[k]) / pd[k];
-            T l2 = (T(sc.bmax[k]) - po[k]) / pd[k];
+            const T inv = T(1) / pd[k];
+            T l1 = (T(sc.bmin[k]) - po[k]) * inv;
+            T l2 = (T(sc.bmax[k]) - po[k]) * inv;
             t_scene_near = tmax(tmin(l1, l2), t_scene_near);
             t_scene_far = tmin(tmax(l1, l2), t_scene_far);
         }
@@ -222,6 +226,7 @@ void trace(const Scene& sc, V3<T> o, V3<T> d, int D, V3<T>& color, T& A, RayOut&
     T t_near = t_scene_near, t_far = t_scene_far;
     std::vector<HitRec<T>> hits;
     const T po[3] = {o.x, o.y, o.z}, pd[3] = {d.x, d.y, d.z};
+    const T pinv[3] = {T(1) / pd[0], T(1) / pd[1], T(1) / pd[2]};
     while (A < T(0.99f)) {
         while ((node.x & 7u) != 3u) {
             const unsigned child = node.x >> 3;
@@ -229,7 +234,7 @@ void trace(const Scene& sc, V3<T> o, V3<T> d, int D, V3<T>& color, T& A, RayOut&
             const unsigned ax = axis > 2 ? 2 : axis; // get_dir_pos: default -> z
             float split;
             std::memcpy(&split, &node.y, 4);
-            const T t_split = (T(split) - po[ax]) / pd[ax];
+            const T t_split = (T(split) - po[ax]) * pinv[ax];
             const unsigned sign = std::signbit(pd[ax]) ? 1u : 0u;
             const unsigned idx = child + (sign ^ (t_split <= t_near ? 1u : 0u));
             if (t_near < t_split && t_split < t_far) {
