@@ -8,6 +8,8 @@ scene resident in HBM, plus
                 traversal+compositing, OpenMP) on this host's cores,
   batched       the 64 mixed-resolution views in one persistent launch
                 (hipGraph replay) against 64 plain launches of the same views,
+  hbm_copy      a 1 GiB streaming copy (k_copy), the measured HBM ceiling and
+                the calibration kernel of the traffic counters,
   build         host SAH build time of the scene.
 """
 import argparse
@@ -82,14 +84,33 @@ def algorithmic_bytes(counters):
 
 
 def load_traffic(workload_key):
+    """HBM bytes per launch of the dominant kernel for `workload_key`, from the
+    calibrated FETCH_SIZE / WRITE_SIZE counter passes summarised in
+    profiles/traffic.json (None when no summary exists)."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(p):
         return None
-    try:
-        with open(p) as f:
-            return json.load(f).get(workload_key)
-    except Exception:
-        return None
+    with open(p) as f:
+        entry = json.load(f).get(workload_key)
+    return None if entry is None else int(entry["hbm_bytes_per_launch"])
+
+
+def copy_bandwidth(dev, stream, gib=1, reps=5):
+    """Streaming copy of `gib` GiB (k_copy): read + write bytes / event time."""
+    import ctypes as C
+    n4 = gib * (1 << 30) // 16
+    src, dst = dev.alloc(n4 * 16), dev.alloc(n4 * 16)
+    K.check(K.lib().kdgrt_memset(dev.h, src.ptr, 0, C.c_size_t(n4 * 16)))
+    K.check(K.lib().kdgrt_copy_f32x4(dev.h, dst.ptr, src.ptr, C.c_uint64(n4), stream))
+    dev.sync()
+    dev.timer_begin(stream)
+    for _ in range(reps):
+        K.check(K.lib().kdgrt_copy_f32x4(dev.h, dst.ptr, src.ptr, C.c_uint64(n4), stream))
+    ms = dev.timer_end(stream) / reps
+    src.free()
+    dst.free()
+    return {"kernel": "k_copy", "bytes_read": n4 * 16, "bytes_written": n4 * 16, "ms": round(ms, 4),
+            "GB_s": round(2 * n4 * 16 / (ms * 1e-3) / 1e9, 1)}
 
 
 def cpu_baseline(tree, g_inv, ci, w, h, budget_s):
@@ -161,7 +182,8 @@ def main():
     log("mean counters/ray: " + ", ".join(f"{n}={v:.2f}" for n, v in zip(K.COUNTER_NAMES, mean_c)))
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": load_traffic("plain_1080p_1M"),
-            "algorithmic_bytes_per_launch": B, "bytes_per_ray": round(B / rays, 1), "kernel": "k_view<12,8,false,0>"}
+            "algorithmic_bytes_per_launch": B, "bytes_per_ray": round(B / rays, 1),
+            "kernel": "k_view<8, false, 0, false, true> (timed with its k_cone_prep)"}
 
     phases = None
     if args.phases:
@@ -181,13 +203,15 @@ def main():
         phases["exact_tests_per_ray"] = round(float(pr[:, 6].mean()), 2)
         log(f"phase shares: {phases}")
 
+    copy = copy_bandwidth(dev, stream)
+    log(f"hbm copy: {copy}")
     result = {"metric": METRIC, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
               "warmup": args.warmup, "ms_per_step": round(ms_launch, 4), "higher_is_better": True, "scaling": "weak",
               "vs_baseline": None, "dtype": "f32", "data": "synthetic",
               "config": {"workload": f"plain one-view launch: {len(g)} clustered Gaussians, one 1920x1080 pinhole "
                                      f"view from radius 25, 1 ray/pixel, short stack 12",
                          "gaussians": len(g), "width": W, "height": H, "stack_depth": 12},
-              "device": info["name"], "roofline": roof,
+              "device": info["name"], "roofline": roof, "hbm_copy": copy,
               **({"phases": phases} if phases else {}),
               "build": {"host_sah_build_s": round(build_s, 2), "threads": os.cpu_count(), "nodes": st["n_nodes"],
                         "leaf_refs": st["n_offsets"], "proxy_triangles": st["n_triangles"]}}
