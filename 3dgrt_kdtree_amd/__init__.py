@@ -458,6 +458,78 @@ class Batch:
             self.h = None
 
 
+class Server:
+    """Serving loop (kdgrt_server_*): resident scenes, HBM-budget admission,
+    continuous batching.  dev=None with simulate_mrays_per_s > 0 runs the same
+    scheduler against a modelled device."""
+
+    def __init__(self, dev, budget_bytes, max_batch_views=16, max_inflight=2, plain=False, tuning=None,
+                 window_ms=1.0, simulate_mrays_per_s=0.0):
+        cfg = ServerConfig()
+        cfg.hbm_budget_bytes = int(budget_bytes)
+        cfg.max_batch_views = int(max_batch_views)
+        cfg.max_inflight_batches = int(max_inflight)
+        cfg.plain_launches = 1 if plain else 0
+        cfg.tuning = tuning if tuning is not None else default_tuning(dev)
+        cfg.batch_window_ms = float(window_ms)
+        cfg.simulate_mrays_per_s = float(simulate_mrays_per_s)
+        self.h = C.c_void_p()
+        check(lib().kdgrt_server_create(dev.h if dev is not None else None, C.byref(cfg), C.byref(self.h)))
+
+    def add_scene(self, tree, g, scales_inverted=False):
+        g = np.ascontiguousarray(g, dtype=GAUSSIAN_DTYPE)
+        sid = C.c_int32()
+        check(lib().kdgrt_server_add_scene(self.h, tree.h, _p(g), C.c_uint64(len(g)),
+                                           C.c_int(1 if scales_inverted else 0), C.byref(sid)))
+        return sid.value
+
+    def add_resident_scene(self, scene):
+        """Serves a device Scene the caller keeps (it must outlive the server)."""
+        sid = C.c_int32()
+        check(lib().kdgrt_server_add_resident_scene(self.h, scene.h, C.byref(sid)))
+        return sid.value
+
+    def start(self):
+        check(lib().kdgrt_server_start(self.h))
+
+    def submit(self, scene_id, cam_info, width, height, arrival_s=0.0, deadline_s=0.0):
+        r = Request()
+        r.scene_id, r.cam, r.width, r.height = scene_id, cam_info, width, height
+        r.arrival_s, r.deadline_s = arrival_s, deadline_s
+        rid = C.c_uint64()
+        check(lib().kdgrt_server_submit(self.h, C.byref(r), C.byref(rid)))
+        return rid.value
+
+    def wait(self, request_id, timeout_ms=-1.0, out=None):
+        """Waits for a request; `out` (float32 array of width*height*4) receives
+        the RGBT image if given.  Returns the ResultInfo."""
+        info = ResultInfo()
+        ptr = None
+        if out is not None:
+            assert out.dtype == np.float32 and out.flags["C_CONTIGUOUS"]
+            ptr = out.ctypes.data_as(C.POINTER(C.c_float))
+        check(lib().kdgrt_server_wait(self.h, C.c_uint64(request_id), C.c_double(timeout_ms), ptr,
+                                      C.byref(info)))
+        return info
+
+    def now(self):
+        return lib().kdgrt_server_now(self.h)
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().kdgrt_server_stats_get(self.h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def stop(self):
+        check(lib().kdgrt_server_stop(self.h))
+
+    def close(self):
+        if self.h:
+            lib().kdgrt_server_stop(self.h)
+            lib().kdgrt_server_destroy(self.h)
+            self.h = None
+
+
 def autotune(dev, views, ray_sets=(), max_trials=24, reps=3):
     nv, ns = len(views), len(ray_sets)
     va = (View * max(1, nv))(*views)

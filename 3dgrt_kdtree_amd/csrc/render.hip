@@ -19,6 +19,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "internal.hpp"
@@ -508,6 +509,21 @@ static float4 bound_sphere(const float4& a, const float4& b, const float4& q) {
     return make_float4(a.x, a.y, a.z, w);
 }
 
+// Runs fn(begin, end) over [0, n) in contiguous chunks on up to 16 host
+// threads (the per-Gaussian repack of a multi-million-Gaussian scene).
+template <class F>
+static void parallel_ranges(uint64_t n, F&& fn) {
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const uint64_t chunks = std::min<uint64_t>(hw, (n + 65535) / 65536);
+    if (chunks <= 1) {
+        fn(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (uint64_t c = 0; c < chunks; ++c) th.emplace_back([&, c] { fn(n * c / chunks, n * (c + 1) / chunks); });
+    for (auto& t : th) t.join();
+}
+
 extern "C" {
 
 kdgrt_status kdgrt_device_count(int* count) {
@@ -663,7 +679,8 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
     // leaf-slot spheres plus kLsphPad zero records: the pre-test reads whole
     // groups of four past a leaf's end and masks the extra bits off
     std::vector<float4> lsph(n_offsets + kLsphPad, make_float4(0.f, 0.f, 0.f, 0.f));
-    for (uint64_t i = 0; i < n_g; ++i) {
+    auto repack = [&](uint64_t i0, uint64_t i1) {
+    for (uint64_t i = i0; i < i1; ++i) {
         const kdgrt_gaussian& s = g[i];
         float inv[3], kinv;
         if (scales_inverted) {
@@ -700,7 +717,11 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
         sph[i] = bound_sphere(make_float4(s.pos[0], s.pos[1], s.pos[2], s.opacity),
                               make_float4(inv[0], inv[1], inv[2], kinv), q);
     }
-    for (uint64_t k = 0; k < n_offsets; ++k) lsph[k] = sph[offsets[k]]; // leaf-slot order
+    };
+    parallel_ranges(n_g, repack);
+    parallel_ranges(n_offsets, [&](uint64_t k0, uint64_t k1) {
+        for (uint64_t k = k0; k < k1; ++k) lsph[k] = sph[offsets[k]]; // leaf-slot order
+    });
     auto* sc = new kdgrt_scene();
     sc->ctx = ctx;
     auto fail = [&](hipError_t e, const char* what) {

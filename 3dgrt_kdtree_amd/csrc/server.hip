@@ -122,6 +122,7 @@ struct kdgrt_server {
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::map<int32_t, std::pair<kdgrt_scene*, uint64_t>> scenes;
+    std::map<int32_t, bool> scene_owned; // false: a caller's resident scene, not destroyed here
     int32_t next_scene = 0;
     uint64_t scene_bytes = 0;
 
@@ -348,6 +349,26 @@ kdgrt_status kdgrt_server_add_scene(kdgrt_server* s, const kdgrt_tree* tree, con
     }
     const int32_t id = s->next_scene++;
     s->scenes[id] = {sc, bytes};
+    s->scene_owned[id] = true;
+    s->scene_bytes += bytes;
+    s->st.bytes_resident = s->scene_bytes;
+    *scene_id = id;
+    return KDGRT_OK;
+}
+
+kdgrt_status kdgrt_server_add_resident_scene(kdgrt_server* s, kdgrt_scene* scene, int32_t* scene_id) {
+    if (!s || !scene || !scene_id) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "server_add_resident_scene: null");
+    if (s->sim) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "server_add_resident_scene: modelled device");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->started) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "server_add_resident_scene: server already started");
+    uint64_t bytes = 0;
+    kdgrt_status st = kdgrt_scene_device_bytes(scene, &bytes);
+    if (st != KDGRT_OK) return st;
+    if (s->scene_bytes + bytes > s->cfg.hbm_budget_bytes)
+        return set_error(KDGRT_ERR_CAPACITY, "server_add_resident_scene: scene does not fit the HBM budget");
+    const int32_t id = s->next_scene++;
+    s->scenes[id] = {scene, bytes};
+    s->scene_owned[id] = false;
     s->scene_bytes += bytes;
     s->st.bytes_resident = s->scene_bytes;
     *scene_id = id;
@@ -360,7 +381,8 @@ kdgrt_status kdgrt_server_remove_scene(kdgrt_server* s, int32_t scene_id) {
     if (s->started) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "server_remove_scene: stop the server first");
     auto it = s->scenes.find(scene_id);
     if (it == s->scenes.end()) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "server_remove_scene: unknown scene");
-    if (it->second.first) kdgrt_scene_destroy(it->second.first);
+    if (it->second.first && s->scene_owned[scene_id]) kdgrt_scene_destroy(it->second.first);
+    s->scene_owned.erase(scene_id);
     s->scene_bytes -= it->second.second;
     s->st.bytes_resident = s->scene_bytes;
     s->scenes.erase(it);
@@ -525,7 +547,7 @@ kdgrt_status kdgrt_server_destroy(kdgrt_server* s) {
         if (s->copy_stream) hipStreamDestroy(s->copy_stream);
         if (s->d_arena) hipFree(s->d_arena);
         for (auto& kv : s->scenes)
-            if (kv.second.first) kdgrt_scene_destroy(kv.second.first);
+            if (kv.second.first && s->scene_owned[kv.first]) kdgrt_scene_destroy(kv.second.first);
     }
     for (auto& kv : s->reqs) delete kv.second;
     delete s;

@@ -113,6 +113,112 @@ def copy_bandwidth(dev, stream, gib=1, reps=5):
             "GB_s": round(2 * n4 * 16 / (ms * 1e-3) / 1e9, 1)}
 
 
+def random_view(rng, aabb):
+    """A request of the batched-view mix: resolution uniform over the four
+    sizes, eye uniform on the radius-25 sphere about the box centre, aim
+    jittered by +-10 degrees, 60 degree FOV."""
+    centre = 0.5 * (aabb[:3].astype(np.float64) + aabb[3:].astype(np.float64))
+    w, h = RESOLUTIONS[rng.integers(0, 4)]
+    d = rng.normal(size=3)
+    d /= np.linalg.norm(d)
+    eye = centre + 25.0 * d
+    fwd = centre - eye
+    fwd /= np.linalg.norm(fwd)
+    yaw, pitch = np.radians(rng.uniform(-10, 10, 2))
+    up = np.array([0.0, 1.0, 0.0]) if abs(fwd[1]) < 0.99 else np.array([0.0, 0.0, 1.0])
+    right = np.cross(fwd, up)
+    right /= np.linalg.norm(right)
+    upv = np.cross(right, fwd)
+    fwd2 = fwd * math.cos(yaw) * math.cos(pitch) + right * math.sin(yaw) + upv * math.sin(pitch)
+    return K.camera_make(K.look_at(eye, eye + fwd2, 60.0, w / h, 0.005), w, h), w, h
+
+
+def run_poisson(srv, ids, aabbs, lam, duration_s, seed):
+    """Poisson arrivals at `lam` requests/s for `duration_s`; each request a
+    random resident scene and a random view of the mix.  Latency is arrival
+    to result ready (server clock)."""
+    import threading
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / lam, size=int(lam * duration_s * 1.5) + 16)
+    arrivals = np.cumsum(gaps)
+    arrivals = arrivals[arrivals < duration_s]
+    reqs = []
+    for _ in range(len(arrivals)):
+        k = int(rng.integers(0, len(ids)))
+        ci, w, h = random_view(rng, aabbs[k])
+        reqs.append((ids[k], ci, w, h))
+    rids = [None] * len(reqs)
+    infos = [None] * len(reqs)
+
+    def waiter():
+        for i in range(len(reqs)):
+            while rids[i] is None:
+                time.sleep(2e-4)
+            infos[i] = srv.wait(rids[i], timeout_ms=-1.0)
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t0 = time.perf_counter()
+    for i, (sid, ci, w, h) in enumerate(reqs):
+        delay = t0 + arrivals[i] - time.perf_counter()
+        if delay > 2e-3:
+            time.sleep(delay - 1e-3)
+        while time.perf_counter() < t0 + arrivals[i]:
+            pass
+        rids[i] = srv.submit(sid, ci, w, h)
+    th.join()
+    lat = np.array([(x.done_s - x.arrival_s) * 1e3 for x in infos])
+    rays = sum(w * h for _, _, w, h in reqs)
+    span = max(x.done_s for x in infos) - min(x.arrival_s for x in infos)
+    views = np.array([x.batch_views for x in infos])
+    return {"requests": len(reqs), "offered_mrays_s": round(rays / duration_s / 1e6, 1),
+            "mrays_s": round(rays / span / 1e6, 1), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3), "max_ms": round(float(lat.max()), 3),
+            "mean_views_per_launch": round(float(views.mean()), 2),
+            "failed": int(sum(1 for x in infos if x.status != 0))}
+
+
+def serve_bench(dev, n_scenes, tuning, rates=((200, 4.0), (1000, 2.0))):
+    """Serving: n_scenes resident scenes (clustered generator, Gaussian counts
+    log-uniform in [100k, 3M]), HBM budget = resident bytes / 0.8, Poisson
+    arrivals, one view per request; continuous batching against plain
+    one-launch-per-request."""
+    rng = np.random.default_rng(1004)
+    counts = np.exp(rng.uniform(np.log(1e5), np.log(3e6), n_scenes)).astype(np.int64)
+    t0 = time.time()
+    scenes, aabbs = [], []
+    for j, n in enumerate(counts):
+        g = K.generate_gaussians(1, int(n), 1004000 + j)
+        tree = K.Tree.build(g)
+        scenes.append(dev.scene(tree, g))
+        aabbs.append(tree.aabb.copy())
+        tree.free()
+        del g
+    setup_s = time.time() - t0
+    resident = sum(sc.device_bytes() for sc in scenes)
+    budget = int(resident / 0.8)
+    log(f"serving: {n_scenes} scenes, {int(counts.sum())} Gaussians, resident {resident / 1e9:.2f} GB, "
+        f"budget {budget / 1e9:.2f} GB, setup {setup_s:.1f} s")
+    out = {"scenes": n_scenes, "gaussians": [int(c) for c in counts], "resident_bytes": resident,
+           "hbm_budget_bytes": budget, "setup_s": round(setup_s, 1)}
+    for mode in ("batched", "plain"):
+        srv = K.Server(dev, budget, max_batch_views=32, max_inflight=3, plain=(mode == "plain"), tuning=tuning,
+                       window_ms=2.0)
+        ids = [srv.add_resident_scene(sc) for sc in scenes]
+        srv.start()
+        for lam, dur in rates:
+            r = run_poisson(srv, ids, aabbs, lam, dur, seed=2000 + lam)
+            out.setdefault(f"poisson_{lam}", {})[mode] = r
+            log(f"serving {mode} at {lam} req/s: {r}")
+        srv.close()
+    for lam, _ in rates:
+        e = out[f"poisson_{lam}"]
+        e["served_vs_plain_mrays"] = round(e["batched"]["mrays_s"] / e["plain"]["mrays_s"], 3)
+    for sc in scenes:
+        sc.free()
+    return out
+
+
 def cpu_baseline(tree, g_inv, ci, w, h, budget_s):
     import oracle_lib as O
     threads = O.lib().oracle_max_threads()
@@ -137,6 +243,8 @@ def main():
     ap.add_argument("--gaussians", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-scenes", type=int, default=16)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--phases", action="store_true", help="add the per-phase cycle profile of the plain kernel")
     args = ap.parse_args()
@@ -255,6 +363,12 @@ def main():
         batch.free()
         for b in bufs:
             b.free()
+
+    if not args.no_serve:
+        tun = K.default_tuning(dev)
+        if "batched" in result:
+            tun = K.Tuning(**result["batched"]["tuning"])
+        result["serving"] = serve_bench(dev, args.serve_scenes, tun)
 
     if not args.no_cpu:
         gi = K.invert_scales(g)

@@ -318,6 +318,11 @@ KDGRT_API kdgrt_status kdgrt_server_create(kdgrt_context* ctx, const kdgrt_serve
 /* Makes a scene resident (counted against the budget); before start only. */
 KDGRT_API kdgrt_status kdgrt_server_add_scene(kdgrt_server* s, const kdgrt_tree* tree, const kdgrt_gaussian* g,
                                               uint64_t n_gaussians, int scales_inverted, int32_t* scene_id);
+/* Serves a scene the caller created with kdgrt_scene_create*: the server
+ * does not destroy it, and it must outlive the server.  Its device bytes
+ * count against the budget like an added scene's.  No reference counterpart
+ * (the reference holds one scene in module globals, cudaRenderer.cu:343-355). */
+KDGRT_API kdgrt_status kdgrt_server_add_resident_scene(kdgrt_server* s, kdgrt_scene* scene, int32_t* scene_id);
 KDGRT_API kdgrt_status kdgrt_server_remove_scene(kdgrt_server* s, int32_t scene_id);
 KDGRT_API kdgrt_status kdgrt_server_start(kdgrt_server* s);
 /* Thread-safe.  Returns KDGRT_ERR_CAPACITY when the request can never fit the
