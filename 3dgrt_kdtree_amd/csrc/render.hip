@@ -101,7 +101,7 @@ __global__ __launch_bounds__(64) void k_view(ViewArgs a) {
         dbg.tlast = clock64();
     }
     extern __shared__ uint2 lds_stack[];
-    trace_ray<K, DBG, PROF, CONE>(a.sc, o, d, a.depth, lds_stack + lane, rgb, A, &dbg);
+    trace_ray<K, DBG, PROF, CONE>(a.sc, o, d, a.depth, lds_stack, lane, rgb, A, &dbg);
     if (PROF) {
         for (int i = 0; i < 5; ++i) a.prof[8 * rid + i] = dbg.cyc[i];
         a.prof[8 * rid + 5] = dbg.pretests;
@@ -139,7 +139,7 @@ struct RaysArgs {
 };
 
 template <int K, bool DBG>
-__global__ __launch_bounds__(64) void k_rays(RaysArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DBG ? 1 : 4))) void k_rays(RaysArgs a) {
     const uint64_t i = (uint64_t)xcd_interleave<64>(blockIdx.x) * 64 + threadIdx.x;
     if (i >= a.n) return;
     const float o[3] = {a.ro[3 * i], a.ro[3 * i + 1], a.ro[3 * i + 2]};
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(64) void k_rays(RaysArgs a) {
         dbg.nseq = 0;
     }
     extern __shared__ uint2 lds_stack[];
-    trace_ray<K, DBG>(a.sc, o, d, a.depth, lds_stack + threadIdx.x, rgb, A, &dbg);
+    trace_ray<K, DBG>(a.sc, o, d, a.depth, lds_stack, (int)threadIdx.x, rgb, A, &dbg);
     a.out[i] = make_float4(rgb[0], rgb[1], rgb[2], 1.0f - A);
     if (DBG) {
         if (a.counters)
@@ -238,7 +238,7 @@ template <int K>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 4 : 1))) void k_batch(BatchArgs a) {
     const int lane = threadIdx.x & 63;
     extern __shared__ uint2 lds_stack[];
-    uint2* const stk = lds_stack + (threadIdx.x >> 6) * (64 * a.depth) + lane;
+    uint2* const wave_lds = lds_stack + (threadIdx.x >> 6) * (wave_lds_bytes(a.depth) / sizeof(uint2));
     const int home = blockIdx.x % kXcds;
     int probe = 0;
     while (probe < kXcds) {
@@ -307,7 +307,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 4 
             }
             if (live) {
                 float rgb[3], A;
-                trace_ray<K, false>(it.sc, o, d, a.depth, stk, rgb, A, nullptr);
+                trace_ray<K, false>(it.sc, o, d, a.depth, wave_lds, lane, rgb, A, nullptr);
                 it.out[dst] = make_float4(rgb[0], rgb[1], rgb[2], 1.0f - A);
             }
         }
@@ -316,9 +316,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 4 
 
 typedef void (*BatchKernel)(BatchArgs);
 
-// Dynamic LDS of a launch: one 8-B short-stack ring entry per lane and level.
+// Dynamic LDS of a launch: per wavefront, one 8-B short-stack ring entry per
+// lane and level plus each lane's 64-B SH basis.
 static inline size_t stack_lds_bytes(int depth, int block_threads) {
-    return (size_t)depth * 8u * (size_t)block_threads;
+    return wave_lds_bytes(depth) * (size_t)(block_threads / 64);
 }
 
 BatchKernel pick_batch_kernel(int k) {
@@ -344,7 +345,7 @@ struct kdgrt_scene {
     DevScene dev{};
     void* d_nodes = nullptr;
     void* d_offs = nullptr;
-    void* d_geom = nullptr;
+    void* d_sgeom = nullptr;
     void* d_shade = nullptr;
     void* d_sph = nullptr;
     void* d_lsph = nullptr;
@@ -660,8 +661,8 @@ kdgrt_status kdgrt_copy_f32x4(kdgrt_context* ctx, void* dst, const void* src, ui
 
 uint64_t kdgrt_scene_bytes_estimate(uint32_t n_nodes, uint64_t n_offsets, uint64_t n_gaussians) {
     auto r = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
-    return r(8ull * n_nodes) + r(4ull * n_offsets) + r(80ull * n_gaussians) + r(192ull * n_gaussians) +
-           r(16ull * n_gaussians) + 2 * r(16ull * (n_offsets + kLsphPad));
+    return r(8ull * n_nodes) + r(4ull * n_offsets) + r(80ull * std::max<uint64_t>(1, n_offsets)) +
+           r(192ull * n_gaussians) + r(16ull * n_gaussians) + 2 * r(16ull * (n_offsets + kLsphPad));
 }
 
 kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uint32_t n_nodes,
@@ -673,8 +674,8 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
     kdgrt_status st = kdgrt::validate_tree(nodes, n_nodes, n_offsets, offsets, n_g);
     if (st != KDGRT_OK) return st;
     HIP_TRY(hipSetDevice(ctx->device));
-    // Repack: 48-B geometry records (pos|opacity, inverse scale|k_scale,
-    // quaternion) and 192-B SH records.
+    // Repack: 80-B geometry records (built per Gaussian, stored per leaf
+    // slot), 192-B SH records, 16-B bound spheres.
     std::vector<float4> geom(5 * n_g), shade(12 * n_g), sph(n_g);
     // leaf-slot spheres plus kLsphPad zero records: the pre-test reads whole
     // groups of four past a leaf's end and masks the extra bits off
@@ -727,7 +728,7 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
     auto fail = [&](hipError_t e, const char* what) {
         if (sc->d_nodes) hipFree(sc->d_nodes);
         if (sc->d_offs) hipFree(sc->d_offs);
-        if (sc->d_geom) hipFree(sc->d_geom);
+        if (sc->d_sgeom) hipFree(sc->d_sgeom);
         if (sc->d_shade) hipFree(sc->d_shade);
         if (sc->d_sph) hipFree(sc->d_sph);
         if (sc->d_lsph) hipFree(sc->d_lsph);
@@ -740,7 +741,6 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
     hipError_t e;
     if ((e = hipMalloc(&sc->d_nodes, 8ull * n_nodes)) != hipSuccess) return fail(e, "nodes");
     if ((e = hipMalloc(&sc->d_offs, std::max<uint64_t>(4, 4ull * n_offsets))) != hipSuccess) return fail(e, "offsets");
-    if ((e = hipMalloc(&sc->d_geom, 80ull * n_g)) != hipSuccess) return fail(e, "geometry");
     if ((e = hipMalloc(&sc->d_shade, 192ull * n_g)) != hipSuccess) return fail(e, "sh");
     if ((e = hipMalloc(&sc->d_sph, 16ull * n_g)) != hipSuccess) return fail(e, "spheres");
     if ((e = hipMalloc(&sc->d_lsph, 16ull * lsph.size())) != hipSuccess) return fail(e, "leaf spheres");
@@ -753,12 +753,26 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
     if ((e = hipMemcpy(sc->d_nodes, nodes, 8ull * n_nodes, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "copy");
     if (n_offsets && (e = hipMemcpy(sc->d_offs, offsets, 4ull * n_offsets, hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "copy");
-    if ((e = hipMemcpy(sc->d_geom, geom.data(), 80ull * n_g, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "copy");
+    {
+        // geometry records in leaf-slot order, the Gaussian ID in the free last word
+        std::vector<float4> sgeom(5 * std::max<uint64_t>(1, n_offsets));
+        parallel_ranges(n_offsets, [&](uint64_t k0, uint64_t k1) {
+            for (uint64_t k = k0; k < k1; ++k) {
+                for (int q = 0; q < 5; ++q) sgeom[5 * k + q] = geom[5 * (uint64_t)offsets[k] + q];
+                float w;
+                std::memcpy(&w, &offsets[k], 4);
+                sgeom[5 * k + 4].w = w;
+            }
+        });
+        if ((e = hipMalloc(&sc->d_sgeom, 16ull * sgeom.size())) != hipSuccess) return fail(e, "slot geometry");
+        if ((e = hipMemcpy(sc->d_sgeom, sgeom.data(), 16ull * sgeom.size(), hipMemcpyHostToDevice)) != hipSuccess)
+            return fail(e, "copy");
+    }
     if ((e = hipMemcpy(sc->d_shade, shade.data(), 192ull * n_g, hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "copy");
     sc->dev.nodes = (const uint2*)sc->d_nodes;
     sc->dev.offs = (const uint32_t*)sc->d_offs;
-    sc->dev.geom = (const float4*)sc->d_geom;
+    sc->dev.sgeom = (const float4*)sc->d_sgeom;
     sc->dev.shade = (const float4*)sc->d_shade;
     sc->dev.sph = (const float4*)sc->d_sph;
     sc->dev.lsph = (const float4*)sc->d_lsph;
@@ -786,7 +800,7 @@ kdgrt_status kdgrt_scene_destroy(kdgrt_scene* sc) {
     hipSetDevice(sc->ctx->device);
     hipFree(sc->d_nodes);
     hipFree(sc->d_offs);
-    hipFree(sc->d_geom);
+    hipFree(sc->d_sgeom);
     hipFree(sc->d_shade);
     hipFree(sc->d_sph);
     hipFree(sc->d_lsph);
@@ -1034,6 +1048,8 @@ kdgrt_status batch_build(kdgrt_context* ctx, const kdgrt_view* views, int n_view
         return set_error(KDGRT_ERR_INVALID_ARGUMENT, "batch_create: block threads must be 64, 128 or 256");
     if (t.waves_per_cu < 1 || t.waves_per_cu > 64)
         return set_error(KDGRT_ERR_INVALID_ARGUMENT, "batch_create: waves per CU must be 1..64");
+    if (stack_lds_bytes(t.stack_depth, t.block_threads) > ctx->prop.sharedMemPerBlock)
+        return set_error(KDGRT_ERR_INVALID_ARGUMENT, "batch_create: stack depth x block size exceeds the LDS of a block");
 
     std::vector<WorkItem> items;
     items.reserve(n_views + n_sets);
@@ -1161,7 +1177,8 @@ kdgrt_status slot_launch(BatchSlot* sl, const kdgrt_view* views, int n, const kd
     if (!sl || n < 1 || n > sl->max_items) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "slot_launch: bad batch");
     if (t.tile_w < 8 || t.tile_h < 8 || t.tile_w % 8 || t.tile_h % 8 || t.tile_w * t.tile_h > 4096 ||
         t.stack_depth < 1 || t.stack_depth > 32 || (t.hit_capacity != 4 && t.hit_capacity != 8 && t.hit_capacity != 16) ||
-        (t.block_threads != 64 && t.block_threads != 128 && t.block_threads != 256) || t.waves_per_cu < 1)
+        (t.block_threads != 64 && t.block_threads != 128 && t.block_threads != 256) || t.waves_per_cu < 1 ||
+        stack_lds_bytes(t.stack_depth, t.block_threads) > sl->ctx->prop.sharedMemPerBlock)
         return set_error(KDGRT_ERR_INVALID_ARGUMENT, "slot_launch: bad tuning");
     uint64_t tiles = 0, nr = 0;
     for (int i = 0; i < n; ++i) {

@@ -17,9 +17,13 @@
 //
 // Machine plan (not the reference's):
 //  * the short stack is the reference's ring (overwrite the oldest when
-//    full, restart from the root when empty) held in LDS, 8 B per entry,
-//    laid out entry-major across the wave's 64 lanes; push and pop are one
-//    LDS access each and cost no registers beyond the ring index;
+//    full, restart from the root when empty) held in LDS, laid out
+//    entry-major across the wave's 64 lanes; an entry holds the pushed node
+//    itself (both children are fetched at each step, the pair is adjacent)
+//    and its t_far, so a pop costs one LDS access and no global load;
+//  * the exact test reads its 80-B geometry record by leaf slot (records are
+//    stored per slot with the Gaussian ID in the last word), so it does not
+//    wait for an ID load first;
 //  * the per-leaf hit list is a fixed-capacity register buffer of K records
 //    kept sorted by (t, leaf slot), which is the stable insertion-sort
 //    order; a leaf with more than K hits is finished by exact re-scans that
@@ -42,16 +46,17 @@
 
 #include "kd_expf.h"
 
+
 namespace kdgrt_dev {
 
 struct DevScene {
     const uint2* __restrict__ nodes;
     const uint32_t* __restrict__ offs;
-    const float4* __restrict__ geom;  // 5 x float4 per Gaussian: pos|opacity, inverse scale, 1/inverse scale, M
     const float4* __restrict__ shade; // 12 x float4 per Gaussian, channel-major (f_dc[c], f_rest[15c..15c+14])
     const float4* __restrict__ sph;   // 1 x float4 per Gaussian: centre, packed (radius, kappa) as 2 x f16
     const float4* __restrict__ lsph;  // the same records in leaf-slot order: lsph[k] = sph[offs[k]]
     const float4* __restrict__ vcone; // per-view cone records in leaf-slot order (views only, else null)
+    const float4* __restrict__ sgeom; // geometry records in leaf-slot order, Gaussian ID in the last word
     float bmin[3];
     float bmax[3];
 };
@@ -356,34 +361,41 @@ __device__ __forceinline__ bool ray_begin(RayState& st, const DevScene& sc, cons
 // Descent from st.node to the next leaf (cudaRenderer.cu:1555-1573) with
 // the ring short stack (:189-210).  stk is this lane's ring in LDS, entry e
 // at stk[64 * e] (entry-major over the wave's 64 lanes).
+// Stack entry e of this lane: node word x at stk[64 e], node word y at
+// stk[64 (depth + e)], t_far at stk[64 (2 depth + e)] (32-bit words, entry-
+// major over the wave's lanes).  An entry holds the pushed node itself, so a
+// pop needs no global load.
 template <bool DBG>
-__device__ __forceinline__ void descend(RayState& st, const DevScene& sc, int depth, uint2* __restrict__ stk,
+__device__ __forceinline__ void descend(RayState& st, const DevScene& sc, int depth, uint32_t* __restrict__ stk,
                                         RayDbg* dbg) {
     while ((st.node.x & 7u) != 3u) {
         const uint32_t child = st.node.x >> 3;
         const uint32_t axis = st.node.x & 3u;
-        // axis 3 cannot occur in an inner node (3 tags a leaf); as in the
-        // reference's get_dir_pos, anything but 0 and 1 selects z
+        // both children (pair-allocated at child, child + 1) in one step
+        const uint2 n0 = sc.nodes[child], n1 = sc.nodes[child + 1];
         const float oa = sel_axis(axis, st.ox, st.oy, st.oz);
         const float ia = sel_axis(axis, st.ix, st.iy, st.iz);
         const float t_split = (__uint_as_float(st.node.y) - oa) * ia;
         const uint32_t sign = (st.signs >> (axis > 2u ? 2u : axis)) & 1u;
-        const uint32_t idx = child + (sign ^ (t_split <= st.t_near ? 1u : 0u));
+        const uint32_t pick = sign ^ (t_split <= st.t_near ? 1u : 0u);
         if (st.t_near < t_split && t_split < st.t_far) {
             if (++st.top == (uint32_t)depth) st.top = 0; // push overwrites the oldest when full
             st.quant = st.quant + 1 < (uint32_t)depth ? st.quant + 1 : (uint32_t)depth;
-            stk[64 * st.top] = make_uint2(child + (sign ^ 1u), __float_as_uint(st.t_far));
+            const uint2 far = sign ? n0 : n1; // child + (sign ^ 1)
+            stk[64 * st.top] = far.x;
+            stk[64 * (depth + st.top)] = far.y;
+            stk[64 * (2 * depth + st.top)] = __float_as_uint(st.t_far);
             st.t_far = t_split;
         }
         if (DBG) dbg->cnt[0]++;
-        st.node = sc.nodes[idx];
+        st.node = pick ? n1 : n0;
     }
 }
 
 // After a leaf (cudaRenderer.cu:1690-1708): termination test, then pop or
 // restart at the root.  Returns true while the ray continues.
 template <bool DBG>
-__device__ __forceinline__ bool next_leaf(RayState& st, const DevScene& sc, int depth, const uint2* __restrict__ stk,
+__device__ __forceinline__ bool next_leaf(RayState& st, const DevScene& sc, int depth, const uint32_t* __restrict__ stk,
                                           RayDbg* dbg) {
     if ((st.A > 0.99f) | (st.t_far >= st.t_scene_far)) return false;
     if (st.quant == 0) {
@@ -392,13 +404,13 @@ __device__ __forceinline__ bool next_leaf(RayState& st, const DevScene& sc, int 
         st.t_far = st.t_scene_far;
         if (DBG) dbg->cnt[7]++;
     } else {
-        const uint2 e = stk[64 * st.top];
+        st.node = make_uint2(stk[64 * st.top], stk[64 * (depth + st.top)]);
+        const float tf = __uint_as_float(stk[64 * (2 * depth + st.top)]);
         if (st.top == 0) st.top = (uint32_t)depth;
         --st.top;
         --st.quant;
-        st.node = sc.nodes[e.x];
         st.t_near = st.t_far;
-        st.t_far = __uint_as_float(e.y);
+        st.t_far = tf;
     }
     return (++st.guard <= kLeafGuard) && (st.A < 0.99f);
 }
@@ -472,12 +484,11 @@ __device__ __forceinline__ void leaf(RayState& st, const DevScene& sc, RayDbg* d
                 const uint32_t j2 = two ? (uint32_t)__builtin_ctzll(mask) : j1;
                 if (two) mask &= mask - 1;
                 const uint32_t s1 = w0 + j1, s2 = w0 + j2;
-                const uint32_t g1 = sc.offs[base + s1];
-                const uint32_t g2 = two ? sc.offs[base + s2] : g1;
-                const float4* r1p = sc.geom + 5 * (size_t)g1;
-                const float4* r2p = sc.geom + 5 * (size_t)g2;
+                const float4* r1p = sc.sgeom + 5 * (size_t)(base + s1);
+                const float4* r2p = sc.sgeom + 5 * (size_t)(base + s2);
                 const float4 a1 = r1p[0], b1 = r1p[1], c1 = r1p[2], e1 = r1p[3], f1 = r1p[4];
                 const float4 a2 = r2p[0], b2 = r2p[1], c2 = r2p[2], e2 = r2p[3], f2 = r2p[4];
+                const uint32_t g1 = __float_as_uint(f1.w), g2 = __float_as_uint(f2.w);
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
                     if (u == 1 && !two) break;
@@ -506,14 +517,15 @@ __device__ __forceinline__ void leaf(RayState& st, const DevScene& sc, RayDbg* d
             const float alpha = fminf(0.99f, ko[i] * gres);
             if (gres < 0.0113f || alpha < 1.0f / 255.0f) continue;
             const uint32_t gid = kid[i];
+            float c[3];
             // The basis depends only on the ray direction; it is formed here,
-            // per blended sample, rather than held in 16 registers per ray.
+            // per blended sample, rather than held in 16 registers or 4 KB of
+            // LDS per wave (either costs occupancy, which this path needs).
             float bx = st.dx, by = st.dy, bz = st.dz;
             asm volatile("" : "+v"(bx), "+v"(by), "+v"(bz));
             const float dv[3] = {bx, by, bz};
             float shs[16];
             sh_basis(dv, shs);
-            float c[3];
             sh_radiance(sc.shade + 12 * (size_t)gid, shs, c);
             const float w = 1.0f - st.A;
             st.rgb[0] = st.rgb[0] + (c[0] * alpha) * w;
@@ -539,12 +551,19 @@ __device__ __forceinline__ void leaf(RayState& st, const DevScene& sc, RayDbg* d
     }
 }
 
+// Per-wavefront LDS of the trace: the lanes' short-stack rings, depth
+// entries of 12 B (node words x, y and t_far, each entry-major over the 64
+// lanes): 768 * depth bytes per wavefront.
+__host__ __device__ constexpr size_t wave_lds_bytes(int depth) { return (size_t)depth * 768u; }
+
 // Whole ray: outer loop of cudaRenderer.cu:1552-1709.  CONE: the ray starts
-// at the view origin that sc.vcone was prepared for.  stk: this lane's ring
-// (entry e at stk[64 * e]).
+// at the view origin that sc.vcone was prepared for.  wave_lds: this wave's
+// LDS (wave_lds_bytes(depth)); lane: the lane index within the wave.
 template <int K, bool DBG, bool PROF = false, bool CONE = false>
 __device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], const float d[3], int depth,
-                                          uint2* __restrict__ stk, float rgb[3], float& A, RayDbg* dbg) {
+                                          uint2* __restrict__ wave_lds, int lane, float rgb[3], float& A,
+                                          RayDbg* dbg) {
+    uint32_t* const stk = reinterpret_cast<uint32_t*>(wave_lds) + lane;
     RayState st;
     if (ray_begin(st, sc, o, d, depth)) {
         while (true) {

@@ -178,29 +178,99 @@ def run_poisson(srv, ids, aabbs, lam, duration_s, seed):
             "failed": int(sum(1 for x in infos if x.status != 0))}
 
 
-def serve_bench(dev, n_scenes, tuning, rates=((200, 4.0), (1000, 2.0))):
-    """Serving: n_scenes resident scenes (clustered generator, Gaussian counts
-    log-uniform in [100k, 3M]), HBM budget = resident bytes / 0.8, Poisson
-    arrivals, one view per request; continuous batching against plain
-    one-launch-per-request."""
-    rng = np.random.default_rng(1004)
+def scene_set(dev, n_scenes, seed=1004):
+    """n_scenes device scenes from the clustered generator with Gaussian
+    counts log-uniform in [100k, 3M] (the multi-scene and serving sets)."""
+    rng = np.random.default_rng(seed)
     counts = np.exp(rng.uniform(np.log(1e5), np.log(3e6), n_scenes)).astype(np.int64)
     t0 = time.time()
     scenes, aabbs = [], []
     for j, n in enumerate(counts):
-        g = K.generate_gaussians(1, int(n), 1004000 + j)
+        g = K.generate_gaussians(1, int(n), seed * 1000 + j)
         tree = K.Tree.build(g)
         scenes.append(dev.scene(tree, g))
         aabbs.append(tree.aabb.copy())
         tree.free()
         del g
-    setup_s = time.time() - t0
+    return scenes, aabbs, counts, time.time() - t0
+
+
+def multi_scene_batch(dev, scenes, aabbs, counts, tuning, n_views=256, n_rays=4_000_000, seed=1003, reps=3):
+    """Multi-scene batch: n_views views of the resolution mix over the scenes
+    plus n_rays incoherent rays (origins uniform in a scene's box, directions
+    uniform on the sphere) split over the scenes, in one persistent launch,
+    against one plain launch per view / ray list, each waited for."""
+    rng = np.random.default_rng(seed)
+    views, plain, bufs = [], [], []
+    rays_total = 0
+    for _ in range(n_views):
+        k = int(rng.integers(0, len(scenes)))
+        ci, w, h = random_view(rng, aabbs[k])
+        b = dev.alloc(w * h * 16)
+        bufs.append(b)
+        views.append(K.View(scenes[k].h, ci, w, h, b.ptr))
+        plain.append(("view", scenes[k], ci, w, h, b))
+        rays_total += w * h
+    sets = []
+    per = n_rays // len(scenes)
+    for k, sc in enumerate(scenes):
+        lo, hi = aabbs[k][:3].astype(np.float64), aabbs[k][3:].astype(np.float64)
+        o = (lo + (hi - lo) * rng.random((per, 3))).astype(np.float32)
+        d = rng.normal(size=(per, 3))
+        d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+        ob, db, out = dev.upload(o), dev.upload(d), dev.alloc(per * 16)
+        bufs += [ob, db, out]
+        sets.append(K.RaySet(sc.h, ob.ptr, db.ptr, per, out.ptr))
+        plain.append(("rays", sc, ob, db, per, out))
+        rays_total += per
+    stream = dev.stream()
+
+    def run_plain():
+        for p in plain:
+            if p[0] == "view":
+                _, sc, ci, w, h, b = p
+                dev.render(sc, ci, w, h, b, stream=stream)
+            else:
+                _, sc, ob, db, n, out = p
+                K.check(K.lib().kdgrt_render_rays(dev.h, sc.h, ob.ptr, db.ptr, K.C.c_uint64(n), 12, out.ptr,
+                                                  stream))
+            dev.sync(stream)
+
+    run_plain()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run_plain()
+    plain_s = (time.perf_counter() - t0) / reps
+    batch = K.Batch(dev, views, sets, tuning)
+    batch.run(stream, use_graph=True)
+    dev.sync(stream)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.run(stream, use_graph=True)
+        dev.sync(stream)
+    batch_s = (time.perf_counter() - t0) / reps
+    batch.free()
+    for b in bufs:
+        b.free()
+    out = {"scenes": len(scenes), "gaussians": [int(c) for c in counts], "views": n_views,
+           "incoherent_rays": per * len(scenes), "rays": rays_total,
+           "plain_mrays_s": round(rays_total / plain_s / 1e6, 2),
+           "batched_graph_mrays_s": round(rays_total / batch_s / 1e6, 2), "speedup": round(plain_s / batch_s, 3),
+           "tuning": tuning.as_dict()}
+    log(f"multi-scene batch: {out}")
+    return out
+
+
+def serve_bench(dev, scenes, aabbs, counts, tuning, rates=((200, 4.0), (1000, 2.0))):
+    """Serving: the resident scenes (HBM budget = resident bytes / 0.8),
+    Poisson arrivals, one view per request; continuous batching against
+    plain one-launch-per-request."""
     resident = sum(sc.device_bytes() for sc in scenes)
     budget = int(resident / 0.8)
-    log(f"serving: {n_scenes} scenes, {int(counts.sum())} Gaussians, resident {resident / 1e9:.2f} GB, "
-        f"budget {budget / 1e9:.2f} GB, setup {setup_s:.1f} s")
-    out = {"scenes": n_scenes, "gaussians": [int(c) for c in counts], "resident_bytes": resident,
-           "hbm_budget_bytes": budget, "setup_s": round(setup_s, 1)}
+    log(f"serving: {len(scenes)} scenes, {int(counts.sum())} Gaussians, resident {resident / 1e9:.2f} GB, "
+        f"budget {budget / 1e9:.2f} GB")
+    out = {"scenes": len(scenes), "gaussians": [int(c) for c in counts], "resident_bytes": resident,
+           "hbm_budget_bytes": budget}
     for mode in ("batched", "plain"):
         srv = K.Server(dev, budget, max_batch_views=32, max_inflight=3, plain=(mode == "plain"), tuning=tuning,
                        window_ms=2.0)
@@ -214,8 +284,6 @@ def serve_bench(dev, n_scenes, tuning, rates=((200, 4.0), (1000, 2.0))):
     for lam, _ in rates:
         e = out[f"poisson_{lam}"]
         e["served_vs_plain_mrays"] = round(e["batched"]["mrays_s"] / e["plain"]["mrays_s"], 3)
-    for sc in scenes:
-        sc.free()
     return out
 
 
@@ -244,6 +312,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-multi", action="store_true", help="skip the multi-scene batch (256 views + 4M rays)")
     ap.add_argument("--serve-scenes", type=int, default=16)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--phases", action="store_true", help="add the per-phase cycle profile of the plain kernel")
@@ -364,11 +433,20 @@ def main():
         for b in bufs:
             b.free()
 
-    if not args.no_serve:
+    if not (args.no_serve and args.no_multi):
         tun = K.default_tuning(dev)
         if "batched" in result:
             tun = K.Tuning(**result["batched"]["tuning"])
-        result["serving"] = serve_bench(dev, args.serve_scenes, tun)
+        scenes, aabbs, counts, setup_s = scene_set(dev, args.serve_scenes)
+        log(f"scene set: {len(scenes)} scenes, {int(counts.sum())} Gaussians, setup {setup_s:.1f} s")
+        if not args.no_multi:
+            m = min(8, len(scenes))
+            result["multi_scene_batch"] = multi_scene_batch(dev, scenes[:m], aabbs[:m], counts[:m], tun)
+        if not args.no_serve:
+            result["serving"] = serve_bench(dev, scenes, aabbs, counts, tun)
+            result["serving"]["setup_s"] = round(setup_s, 1)
+        for sc in scenes:
+            sc.free()
 
     if not args.no_cpu:
         gi = K.invert_scales(g)
