@@ -347,7 +347,6 @@ struct kdgrt_scene {
     void* d_offs = nullptr;
     void* d_sgeom = nullptr;
     void* d_shade = nullptr;
-    void* d_sph = nullptr;
     void* d_lsph = nullptr;
     // Cone records of the plain view launches, one table per scene: launches
     // on one stream run in order; a launch on another stream first waits for
@@ -662,7 +661,7 @@ kdgrt_status kdgrt_copy_f32x4(kdgrt_context* ctx, void* dst, const void* src, ui
 uint64_t kdgrt_scene_bytes_estimate(uint32_t n_nodes, uint64_t n_offsets, uint64_t n_gaussians) {
     auto r = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
     return r(8ull * n_nodes) + r(4ull * n_offsets) + r(80ull * std::max<uint64_t>(1, n_offsets)) +
-           r(192ull * n_gaussians) + r(16ull * n_gaussians) + 2 * r(16ull * (n_offsets + kLsphPad));
+           r(192ull * n_gaussians) + 2 * r(16ull * (n_offsets + kLsphPad));
 }
 
 kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uint32_t n_nodes,
@@ -730,7 +729,6 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
         if (sc->d_offs) hipFree(sc->d_offs);
         if (sc->d_sgeom) hipFree(sc->d_sgeom);
         if (sc->d_shade) hipFree(sc->d_shade);
-        if (sc->d_sph) hipFree(sc->d_sph);
         if (sc->d_lsph) hipFree(sc->d_lsph);
         if (sc->d_vcone) hipFree(sc->d_vcone);
         if (sc->cone_ev) hipEventDestroy(sc->cone_ev);
@@ -742,13 +740,10 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
     if ((e = hipMalloc(&sc->d_nodes, 8ull * n_nodes)) != hipSuccess) return fail(e, "nodes");
     if ((e = hipMalloc(&sc->d_offs, std::max<uint64_t>(4, 4ull * n_offsets))) != hipSuccess) return fail(e, "offsets");
     if ((e = hipMalloc(&sc->d_shade, 192ull * n_g)) != hipSuccess) return fail(e, "sh");
-    if ((e = hipMalloc(&sc->d_sph, 16ull * n_g)) != hipSuccess) return fail(e, "spheres");
     if ((e = hipMalloc(&sc->d_lsph, 16ull * lsph.size())) != hipSuccess) return fail(e, "leaf spheres");
     if ((e = hipMalloc(&sc->d_vcone, 16ull * lsph.size())) != hipSuccess) return fail(e, "view cones");
     if ((e = hipEventCreateWithFlags(&sc->cone_ev, hipEventDisableTiming)) != hipSuccess) return fail(e, "event");
     if ((e = hipMemcpy(sc->d_lsph, lsph.data(), 16ull * lsph.size(), hipMemcpyHostToDevice)) != hipSuccess)
-        return fail(e, "copy");
-    if ((e = hipMemcpy(sc->d_sph, sph.data(), 16ull * n_g, hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "copy");
     if ((e = hipMemcpy(sc->d_nodes, nodes, 8ull * n_nodes, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "copy");
     if (n_offsets && (e = hipMemcpy(sc->d_offs, offsets, 4ull * n_offsets, hipMemcpyHostToDevice)) != hipSuccess)
@@ -774,7 +769,6 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
     sc->dev.offs = (const uint32_t*)sc->d_offs;
     sc->dev.sgeom = (const float4*)sc->d_sgeom;
     sc->dev.shade = (const float4*)sc->d_shade;
-    sc->dev.sph = (const float4*)sc->d_sph;
     sc->dev.lsph = (const float4*)sc->d_lsph;
     for (int k = 0; k < 3; ++k) {
         sc->dev.bmin[k] = aabb[k];
@@ -802,7 +796,6 @@ kdgrt_status kdgrt_scene_destroy(kdgrt_scene* sc) {
     hipFree(sc->d_offs);
     hipFree(sc->d_sgeom);
     hipFree(sc->d_shade);
-    hipFree(sc->d_sph);
     hipFree(sc->d_lsph);
     hipFree(sc->d_vcone);
     hipEventDestroy(sc->cone_ev);

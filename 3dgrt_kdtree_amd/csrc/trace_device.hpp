@@ -53,8 +53,7 @@ struct DevScene {
     const uint2* __restrict__ nodes;
     const uint32_t* __restrict__ offs;
     const float4* __restrict__ shade; // 12 x float4 per Gaussian, channel-major (f_dc[c], f_rest[15c..15c+14])
-    const float4* __restrict__ sph;   // 1 x float4 per Gaussian: centre, packed (radius, kappa) as 2 x f16
-    const float4* __restrict__ lsph;  // the same records in leaf-slot order: lsph[k] = sph[offs[k]]
+    const float4* __restrict__ lsph;  // bound spheres in leaf-slot order: centre, packed (radius, kappa) as 2 x f16
     const float4* __restrict__ vcone; // per-view cone records in leaf-slot order (views only, else null)
     const float4* __restrict__ sgeom; // geometry records in leaf-slot order, Gaussian ID in the last word
     float bmin[3];

@@ -45,11 +45,18 @@ def _requests(hss, n, seed):
     return out
 
 
-def _serve(dev, hss, reqs, plain, max_views=6, inflight=3):
-    budget = sum(K.lib().kdgrt_scene_bytes_estimate(len(h.nodes), len(h.offsets), len(h.g)) for h in hss)
+def _serve(dev, hss, reqs, plain, max_views=6, inflight=3, resident=None):
+    """resident: device Scenes to serve instead of uploading the host trees."""
+    if resident is None:
+        budget = sum(K.lib().kdgrt_scene_bytes_estimate(len(h.nodes), len(h.offsets), len(h.g)) for h in hss)
+    else:
+        budget = sum(sc.device_bytes() for sc in resident)
     srv = K.Server(dev, budget + (64 << 20), max_batch_views=max_views, max_inflight=inflight, plain=plain,
                    window_ms=1.0)
-    ids = [srv.add_scene(h.tree, h.g) for h in hss]
+    if resident is None:
+        ids = [srv.add_scene(h.tree, h.g) for h in hss]
+    else:
+        ids = [srv.add_resident_scene(sc) for sc in resident]
     srv.start()
     results = [None] * len(reqs)
     rids = [None] * len(reqs)
@@ -128,3 +135,19 @@ def test_served_plain_mode_equals_alone(dev, three_scenes):
         assert info.batch_views == 1
         assert np.array_equal(img, ref), f"request {i}: served != alone"
     assert st["completed"] == len(reqs)
+
+
+def test_served_resident_scenes_equal_alone(dev, three_scenes):
+    """Scenes the caller keeps resident (kdgrt_server_add_resident_scene)
+    serve the same images; the server does not free them."""
+    hss = three_scenes
+    scs = [dev.scene(h.tree, h.g) for h in hss]
+    reqs = _requests(hss, 16, 13)
+    print("[server] 16 requests, resident scenes", flush=True)
+    results, st = _serve(dev, hss, reqs, plain=False, resident=scs)
+    for i, ((img, info), (k, ci, w, h)) in enumerate(zip(results, reqs)):
+        assert info.status == 0
+        assert np.array_equal(img, dev.render_image(scs[k], ci, w, h)), f"request {i}: served != alone"
+    assert st["completed"] == len(reqs)
+    for sc in scs:
+        sc.free()
