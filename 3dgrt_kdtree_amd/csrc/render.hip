@@ -171,6 +171,8 @@ struct WorkItem { // one view or one explicit ray set
     CamInfo cam;
     int32_t W, H;       // view size (ray sets: W = 0)
     int32_t tiles_x;    // view tiles per row
+    int32_t tiles_y;    // view tile rows
+    int32_t groups_x;   // 8x8-tile groups per row (view tiles are numbered group-major)
     uint32_t tile_begin;
     uint32_t tile_count;
     float4* out;
@@ -179,10 +181,27 @@ struct WorkItem { // one view or one explicit ray set
     uint64_t n_rays;
 };
 
+// Tile queues: the tile space is cut into groups of kQueueGroup consecutive
+// tiles (for a view, one 8x8-tile square), dealt round-robin to the kXcds
+// queues; q_len[x] is the number of tiles queue x owns.
+constexpr uint32_t kQueueGroup = 64;
+
+__host__ __device__ __forceinline__ uint32_t queue_tile(uint32_t q, uint32_t j) {
+    return ((j / kQueueGroup) * kXcds + q) * kQueueGroup + (j % kQueueGroup);
+}
+
+static inline void queue_lengths(uint64_t tiles, uint32_t q_len[kXcds]) {
+    const uint64_t groups = tiles / kQueueGroup, rest = tiles % kQueueGroup;
+    for (uint32_t q = 0; q < kXcds; ++q) {
+        const uint64_t full = groups / kXcds + (q < groups % kXcds ? 1 : 0);
+        q_len[q] = (uint32_t)(full * kQueueGroup + ((rest && groups % kXcds == q) ? rest : 0));
+    }
+}
+
 struct BatchArgs {
     const WorkItem* items;
     int n_items;
-    uint32_t q_begin[kXcds + 1]; // queue x owns tiles [q_begin[x], q_begin[x+1])
+    uint32_t q_len[kXcds];       // tiles owned by each queue (queue_tile maps them)
     uint32_t* counters;          // kXcds counters
     int depth;
     int tile_w, tile_h;          // view tile in pixels; tile_w*tile_h is a multiple of 64
@@ -243,7 +262,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 4 
     int probe = 0;
     while (probe < kXcds) {
         const int q = (home + probe) % kXcds;
-        const uint32_t qlen = a.q_begin[q + 1] - a.q_begin[q];
+        const uint32_t qlen = a.q_len[q];
         uint32_t t = 0;
         if (lane == 0) t = atomicAdd(&a.counters[q], 1u);
         // readfirstlane makes the tile (and everything derived from it)
@@ -253,7 +272,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 4 
             ++probe; // this queue is drained; move on for good
             continue;
         }
-        const uint32_t tile = a.q_begin[q] + t;
+        const uint32_t tile = queue_tile((uint32_t)q, t);
         // locate the work item (items are sorted by tile_begin)
         int lo = 0, hi = a.n_items - 1;
         while (lo < hi) {
@@ -272,8 +291,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 4 
         }
         const uint32_t local = tile - it.tile_begin;
         const bool is_view = it.W > 0;
-        const int tx = is_view ? (int)(local % (uint32_t)it.tiles_x) : 0;
-        const int ty = is_view ? (int)(local / (uint32_t)it.tiles_x) : 0;
+        // view tiles are numbered in 8x8-tile groups, row-major within a group
+        const uint32_t grp = local / 64u, gl = local % 64u;
+        const int tx = is_view ? (int)((grp % (uint32_t)it.groups_x) * 8u + gl % 8u) : 0;
+        const int ty = is_view ? (int)((grp / (uint32_t)it.groups_x) * 8u + gl / 8u) : 0;
+        if (is_view && (tx >= it.tiles_x || ty >= it.tiles_y)) continue; // group padding past the view
         const int npx = a.tile_w * a.tile_h;
         for (int p0 = 0; p0 < npx; p0 += 64) {
             const int p = p0 + lane;
@@ -1059,9 +1081,10 @@ kdgrt_status batch_build(kdgrt_context* ctx, const kdgrt_view* views, int n_view
         w.W = v.width;
         w.H = v.height;
         w.tiles_x = (v.width + t.tile_w - 1) / t.tile_w;
-        const uint32_t ty = (uint32_t)((v.height + t.tile_h - 1) / t.tile_h);
+        w.tiles_y = (v.height + t.tile_h - 1) / t.tile_h;
+        w.groups_x = (w.tiles_x + 7) / 8;
         w.tile_begin = (uint32_t)tiles;
-        w.tile_count = (uint32_t)w.tiles_x * ty;
+        w.tile_count = (uint32_t)w.groups_x * (uint32_t)((w.tiles_y + 7) / 8) * 64u;
         w.out = (float4*)v.out_rgbt;
         tiles += w.tile_count;
         rays += (uint64_t)v.width * v.height;
@@ -1108,7 +1131,7 @@ kdgrt_status batch_build(kdgrt_context* ctx, const kdgrt_view* views, int n_view
     BatchArgs& a = b->args;
     a.items = b->d_items;
     a.n_items = (int)items.size();
-    for (int x = 0; x <= kXcds; ++x) a.q_begin[x] = (uint32_t)((uint64_t)tiles * x / kXcds);
+    queue_lengths(tiles, a.q_len);
     a.counters = b->d_counters;
     a.depth = t.stack_depth;
     a.tile_w = t.tile_w;
@@ -1182,8 +1205,10 @@ kdgrt_status slot_launch(BatchSlot* sl, const kdgrt_view* views, int n, const kd
         w.W = v.width;
         w.H = v.height;
         w.tiles_x = (v.width + t.tile_w - 1) / t.tile_w;
+        w.tiles_y = (v.height + t.tile_h - 1) / t.tile_h;
+        w.groups_x = (w.tiles_x + 7) / 8;
         w.tile_begin = (uint32_t)tiles;
-        w.tile_count = (uint32_t)w.tiles_x * (uint32_t)((v.height + t.tile_h - 1) / t.tile_h);
+        w.tile_count = (uint32_t)w.groups_x * (uint32_t)((w.tiles_y + 7) / 8) * 64u;
         w.out = (float4*)v.out_rgbt;
         tiles += w.tile_count;
         nr += (uint64_t)v.width * v.height;
@@ -1193,7 +1218,7 @@ kdgrt_status slot_launch(BatchSlot* sl, const kdgrt_view* views, int n, const kd
     BatchArgs a{};
     a.items = sl->d_items;
     a.n_items = n;
-    for (int x = 0; x <= kXcds; ++x) a.q_begin[x] = (uint32_t)((uint64_t)tiles * x / kXcds);
+    queue_lengths(tiles, a.q_len);
     a.counters = sl->d_counters;
     a.depth = t.stack_depth;
     a.tile_w = t.tile_w;
