@@ -16,6 +16,9 @@
 //  * No allocation and no implicit device synchronisation happen on the
 //    launch path: slots (work table, counters, pinned staging) and streams
 //    are created at start.
+//  * A batch that holds a single view runs the one-view kernel (k_view with
+//    its cone pre-test) on the slot's stream; larger batches run the
+//    persistent kernel.
 //  * plain_launches = 1 is the comparison mode: each request is one plain
 //    one-view launch (k_view), run one at a time, as the reference renders.
 //  * With ctx == NULL and simulate_mrays_per_s > 0 the same scheduler runs
@@ -259,7 +262,11 @@ bool launch_one(kdgrt_server* s) {
             v.out_rgbt = (float*)((char*)s->d_arena + r->off);
         }
         hipStream_t st = s->streams[slot];
-        if (s->cfg.plain_launches) {
+        if (s->cfg.plain_launches || views.size() == 1) {
+            // comparison mode, or a batch of one view: the one-view kernel
+            // (its per-view cone pre-test makes it the faster path for a
+            // single view); it still runs on the slot's stream and overlaps
+            // with the other slots
             stt = kdgrt_render(s->ctx, views[0].scene, &views[0].cam, views[0].width, views[0].height,
                                s->cfg.tuning.stack_depth, KDGRT_OUT_RGBT_F32, views[0].out_rgbt, st);
         } else {

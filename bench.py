@@ -261,7 +261,7 @@ def multi_scene_batch(dev, scenes, aabbs, counts, tuning, n_views=256, n_rays=4_
     return out
 
 
-def serve_bench(dev, scenes, aabbs, counts, tuning, rates=((200, 4.0), (1000, 2.0))):
+def serve_bench(dev, scenes, aabbs, counts, tuning, rates=((200, 4.0), (1000, 2.0)), window_ms=0.0):
     """Serving: the resident scenes (HBM budget = resident bytes / 0.8),
     Poisson arrivals, one view per request; continuous batching against
     plain one-launch-per-request."""
@@ -270,10 +270,10 @@ def serve_bench(dev, scenes, aabbs, counts, tuning, rates=((200, 4.0), (1000, 2.
     log(f"serving: {len(scenes)} scenes, {int(counts.sum())} Gaussians, resident {resident / 1e9:.2f} GB, "
         f"budget {budget / 1e9:.2f} GB")
     out = {"scenes": len(scenes), "gaussians": [int(c) for c in counts], "resident_bytes": resident,
-           "hbm_budget_bytes": budget}
+           "hbm_budget_bytes": budget, "batch_window_ms": window_ms, "max_batch_views": 32, "inflight_batches": 3}
     for mode in ("batched", "plain"):
         srv = K.Server(dev, budget, max_batch_views=32, max_inflight=3, plain=(mode == "plain"), tuning=tuning,
-                       window_ms=2.0)
+                       window_ms=window_ms)
         ids = [srv.add_resident_scene(sc) for sc in scenes]
         srv.start()
         for lam, dur in rates:
@@ -314,6 +314,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-multi", action="store_true", help="skip the multi-scene batch (256 views + 4M rays)")
     ap.add_argument("--serve-scenes", type=int, default=16)
+    ap.add_argument("--serve-window-ms", type=float, default=0.0,
+                    help="how long a busy device waits to fill a batch (0: launch as soon as a slot is free)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--phases", action="store_true", help="add the per-phase cycle profile of the plain kernel")
     args = ap.parse_args()
@@ -443,7 +445,7 @@ def main():
             m = min(8, len(scenes))
             result["multi_scene_batch"] = multi_scene_batch(dev, scenes[:m], aabbs[:m], counts[:m], tun)
         if not args.no_serve:
-            result["serving"] = serve_bench(dev, scenes, aabbs, counts, tun)
+            result["serving"] = serve_bench(dev, scenes, aabbs, counts, tun, window_ms=args.serve_window_ms)
             result["serving"]["setup_s"] = round(setup_s, 1)
         for sc in scenes:
             sc.free()
