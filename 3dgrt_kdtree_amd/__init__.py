@@ -244,6 +244,23 @@ class Tree:
             pass
 
 
+def cameras_from_nerf_json(path, aspect):
+    """Camera list of a NeRF transforms.json (kdgrt_cameras_from_nerf_json)."""
+    n = C.c_int()
+    check(lib().kdgrt_cameras_from_nerf_json(path.encode(), C.c_double(aspect), None, C.c_int(0), C.byref(n)))
+    arr = (Camera * max(1, n.value))()
+    check(lib().kdgrt_cameras_from_nerf_json(path.encode(), C.c_double(aspect), arr, C.c_int(n.value), C.byref(n)))
+    return list(arr)[:n.value]
+
+
+def write_stats(counters, width, height, directory):
+    """Writes the reference's per-ray statistics files from render_debug
+    counters (shape (height, width, 8) or (height*width, 8), ray order)."""
+    c = np.ascontiguousarray(counters, dtype=np.int32).reshape(-1, N_COUNTERS)
+    assert len(c) == width * height
+    check(lib().kdgrt_stats_write(_p(c), C.c_int(width), C.c_int(height), str(directory).encode()))
+
+
 def camera_make(cam, width, height):
     ci = CameraInfo()
     check(lib().kdgrt_camera_make(C.byref(cam), C.c_int(width), C.c_int(height), C.byref(ci)))

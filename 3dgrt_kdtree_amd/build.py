@@ -23,7 +23,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
-HOST_SOURCES = ["host_core.cpp", "sah_builder.cpp"]
+HOST_SOURCES = ["host_core.cpp", "sah_builder.cpp", "ingest_stats.cpp"]
 HIP_SOURCES = ["render.hip", "autotune.hip", "server.hip", "host_api.hip"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fvisibility=hidden", "-I" + INC, "-I" + CSRC]

@@ -1,0 +1,319 @@
+// Host products around the render path:
+//  * NeRF transforms.json camera lists (loadCameraJson,
+//    Kd-treeConverter_v0.9/Kd-treeConverter/LoadCamera.hpp:25-62);
+//  * the per-ray statistics files of HIT_AND_NODE_COUNT_DEBUG renders
+//    (cudaRenderer.cu:3173-3400): renderStats.txt, render_stats.csv,
+//    heatmap_*.csv, isectCnt_kd.bin, blendCnt_kd.bin.
+#include <algorithm>
+#include <cctype>
+#include <cerrno>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+using kdgrt::set_error;
+
+namespace {
+
+// Minimal JSON reader: enough of RFC 8259 to walk objects and arrays and read
+// numbers; strings are skipped over (escapes honoured) and compared raw.
+struct Json {
+    enum Kind { Null, Bool, Num, Str, Arr, Obj } kind = Null;
+    double num = 0;
+    std::string str;
+    std::vector<Json> items;                              // array elements / object values
+    std::vector<std::string> keys;                        // object keys
+    const Json* get(const char* k) const {
+        for (size_t i = 0; i < keys.size(); ++i)
+            if (keys[i] == k) return &items[i];
+        return nullptr;
+    }
+};
+
+class Parser {
+  public:
+    Parser(const char* p, const char* e) : p_(p), e_(e) {}
+    bool parse(Json& out) {
+        if (!value(out, 0)) return false;
+        ws();
+        return p_ == e_;
+    }
+    std::string error;
+
+  private:
+    const char *p_, *e_;
+    void ws() {
+        while (p_ < e_ && std::isspace((unsigned char)*p_)) ++p_;
+    }
+    bool fail(const char* m) {
+        if (error.empty()) error = m;
+        return false;
+    }
+    bool lit(const char* s) {
+        const size_t n = std::strlen(s);
+        if ((size_t)(e_ - p_) < n || std::strncmp(p_, s, n) != 0) return fail("bad literal");
+        p_ += n;
+        return true;
+    }
+    bool string(std::string& s) {
+        if (p_ >= e_ || *p_ != '"') return fail("expected string");
+        ++p_;
+        while (p_ < e_ && *p_ != '"') {
+            if (*p_ == '\\') {
+                if (++p_ >= e_) return fail("bad escape");
+                if (*p_ == 'u') {
+                    if (e_ - p_ < 5) return fail("bad escape");
+                    s += '?';
+                    p_ += 5;
+                    continue;
+                }
+                const char c = *p_;
+                s += c == 'n' ? '\n' : c == 't' ? '\t' : c == 'r' ? '\r' : c == 'b' ? '\b' : c == 'f' ? '\f' : c;
+                ++p_;
+                continue;
+            }
+            s += *p_++;
+        }
+        if (p_ >= e_) return fail("unterminated string");
+        ++p_;
+        return true;
+    }
+    bool value(Json& v, int depth) {
+        if (depth > 64) return fail("nesting too deep");
+        ws();
+        if (p_ >= e_) return fail("unexpected end");
+        const char c = *p_;
+        if (c == '{') {
+            v.kind = Json::Obj;
+            ++p_;
+            ws();
+            if (p_ < e_ && *p_ == '}') {
+                ++p_;
+                return true;
+            }
+            while (true) {
+                ws();
+                std::string k;
+                if (!string(k)) return false;
+                ws();
+                if (p_ >= e_ || *p_ != ':') return fail("expected ':'");
+                ++p_;
+                v.keys.push_back(k);
+                v.items.emplace_back();
+                if (!value(v.items.back(), depth + 1)) return false;
+                ws();
+                if (p_ < e_ && *p_ == ',') {
+                    ++p_;
+                    continue;
+                }
+                if (p_ < e_ && *p_ == '}') {
+                    ++p_;
+                    return true;
+                }
+                return fail("expected ',' or '}'");
+            }
+        }
+        if (c == '[') {
+            v.kind = Json::Arr;
+            ++p_;
+            ws();
+            if (p_ < e_ && *p_ == ']') {
+                ++p_;
+                return true;
+            }
+            while (true) {
+                v.items.emplace_back();
+                if (!value(v.items.back(), depth + 1)) return false;
+                ws();
+                if (p_ < e_ && *p_ == ',') {
+                    ++p_;
+                    continue;
+                }
+                if (p_ < e_ && *p_ == ']') {
+                    ++p_;
+                    return true;
+                }
+                return fail("expected ',' or ']'");
+            }
+        }
+        if (c == '"') {
+            v.kind = Json::Str;
+            return string(v.str);
+        }
+        if (c == 't') {
+            v.kind = Json::Bool;
+            v.num = 1;
+            return lit("true");
+        }
+        if (c == 'f') {
+            v.kind = Json::Bool;
+            return lit("false");
+        }
+        if (c == 'n') return lit("null");
+        // number
+        std::string tok;
+        while (p_ < e_ && (std::isdigit((unsigned char)*p_) || *p_ == '-' || *p_ == '+' || *p_ == '.' || *p_ == 'e' ||
+                           *p_ == 'E'))
+            tok += *p_++;
+        if (tok.empty()) return fail("unexpected character");
+        char* end = nullptr;
+        errno = 0;
+        v.num = std::strtod(tok.c_str(), &end);
+        if (end != tok.c_str() + tok.size() || errno == ERANGE) return fail("bad number");
+        v.kind = Json::Num;
+        return true;
+    }
+};
+
+bool read_file(const char* path, std::string& out) {
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) return false;
+    std::fseek(fp, 0, SEEK_END);
+    const long n = std::ftell(fp);
+    std::fseek(fp, 0, SEEK_SET);
+    out.resize(n > 0 ? (size_t)n : 0);
+    const bool ok = n <= 0 || std::fread(&out[0], 1, (size_t)n, fp) == (size_t)n;
+    std::fclose(fp);
+    return ok;
+}
+
+} // namespace
+
+extern "C" {
+
+kdgrt_status kdgrt_cameras_from_nerf_json(const char* path, double aspect, kdgrt_camera* out, int capacity,
+                                          int* count) {
+    if (!path || !count || (capacity > 0 && !out) || capacity < 0)
+        return set_error(KDGRT_ERR_INVALID_ARGUMENT, "cameras_from_nerf_json: null argument");
+    std::string text;
+    if (!read_file(path, text)) return set_error(KDGRT_ERR_IO, std::string("cameras_from_nerf_json: cannot read ") + path);
+    Json root;
+    Parser ps(text.data(), text.data() + text.size());
+    if (!ps.parse(root)) return set_error(KDGRT_ERR_FORMAT, "cameras_from_nerf_json: " + ps.error);
+    const Json* frames = root.kind == Json::Obj ? root.get("frames") : nullptr;
+    if (!frames || frames->kind != Json::Arr)
+        return set_error(KDGRT_ERR_FORMAT, "cameras_from_nerf_json: no \"frames\" array");
+    int n = 0;
+    for (const Json& f : frames->items) {
+        const Json* m = f.kind == Json::Obj ? f.get("transform_matrix") : nullptr;
+        if (!m || m->kind != Json::Arr || m->items.size() < 3)
+            return set_error(KDGRT_ERR_FORMAT, "cameras_from_nerf_json: frame " + std::to_string(n) +
+                                                   " has no 3x4 transform_matrix");
+        float a[3][4];
+        for (int r = 0; r < 3; ++r) {
+            const Json& row = m->items[r];
+            if (row.kind != Json::Arr || row.items.size() < 4)
+                return set_error(KDGRT_ERR_FORMAT, "cameras_from_nerf_json: short matrix row");
+            for (int c = 0; c < 4; ++c) {
+                if (row.items[c].kind != Json::Num)
+                    return set_error(KDGRT_ERR_FORMAT, "cameras_from_nerf_json: non-numeric matrix entry");
+                a[r][c] = (float)row.items[c].num;
+            }
+        }
+        if (n < capacity) {
+            // columns 0, 1, 2 are the u, v, n axes and column 3 the position;
+            // fovy 39.6 degrees and near 0.005 are the loader's constants
+            kdgrt_camera& cam = out[n];
+            for (int k = 0; k < 3; ++k) {
+                cam.u[k] = a[k][0];
+                cam.v[k] = a[k][1];
+                cam.n[k] = a[k][2];
+                cam.pos[k] = a[k][3];
+            }
+            cam.fovy_deg = 39.6;
+            cam.aspect = aspect;
+            cam.near_c = 0.005;
+        }
+        ++n;
+    }
+    *count = n;
+    return KDGRT_OK;
+}
+
+kdgrt_status kdgrt_stats_write(const int32_t* counters, int width, int height, const char* dir) {
+    if (!counters || !dir || width <= 0 || height <= 0)
+        return set_error(KDGRT_ERR_INVALID_ARGUMENT, "stats_write: bad argument");
+    const size_t np = (size_t)width * height;
+    std::string base(dir);
+    if (!base.empty() && base.back() != '/') base += '/';
+    // counters: 8 per ray in ray order y * width + x (the reference's debug
+    // buffer order); 0 node visits, 1 leaf visits, 2 tests, 3 hits, 4 blends,
+    // 5 max sort size
+    int mx[6] = {0, 0, 0, 0, 0, 0};
+    double tot[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t blending = 0;
+    std::vector<uint64_t> hist_node(1024, 0), hist_leaf(256, 0);
+    auto add = [](std::vector<uint64_t>& h, int v) {
+        if (v < 0) return;
+        if ((size_t)v >= h.size()) h.resize((size_t)v + 1, 0);
+        h[v]++;
+    };
+    for (size_t i = 0; i < np; ++i) {
+        const int32_t* c = counters + 8 * i;
+        for (int k = 0; k < 6; ++k) mx[k] = std::max(mx[k], c[k]);
+        if (c[4] != 0) { // averages and histograms over rays that blended
+            for (int k = 0; k < 6; ++k) tot[k] += c[k];
+            ++blending;
+            add(hist_node, c[0]);
+            add(hist_leaf, c[1]);
+        }
+    }
+    auto open = [&](const char* name, const char* mode) { return std::fopen((base + name).c_str(), mode); };
+    FILE* fp = open("renderStats.txt", "w");
+    if (!fp) return set_error(KDGRT_ERR_IO, "stats_write: cannot write " + base + "renderStats.txt");
+    const double b = (double)blending;
+    const char* names[6] = {"node_visits\t\t", "leaf_visits\t\t", "intersection_tests\t", "hits_found\t\t",
+                            "blend_ops\t\t", "max_sort_size\t"};
+    std::fprintf(fp, "\n===========================================\n");
+    for (int k = 0; k < 6; ++k) std::fprintf(fp, "avg_%s: %f\n", names[k], tot[k] / b);
+    std::fprintf(fp, "===========================================\n");
+    const char* mnames[6] = {"node_visits\t\t", "leaf_visits\t\t", "intersection_tests\t", "hits_found\t\t",
+                             "blend_ops\t\t", "max_sort_size\t"};
+    for (int k = 0; k < 6; ++k) std::fprintf(fp, "max_%s: %d\n", mnames[k], mx[k]);
+    std::fprintf(fp, "===========================================\n");
+    for (int k = 0; k < 6; ++k) std::fprintf(fp, "tot_%s: %f\n", names[k], tot[k]);
+    std::fprintf(fp, "===========================================\n");
+    std::fclose(fp);
+
+    fp = open("render_stats.csv", "w");
+    if (!fp) return set_error(KDGRT_ERR_IO, "stats_write: cannot write render_stats.csv");
+    std::fprintf(fp, "Value,Node_Visits,Leaf_Visits\n");
+    const size_t range = std::max(hist_node.size(), hist_leaf.size());
+    for (size_t v = 0; v < range; ++v) {
+        const uint64_t a = v < hist_node.size() ? hist_node[v] : 0, c = v < hist_leaf.size() ? hist_leaf[v] : 0;
+        if (a || c) std::fprintf(fp, "%zu,%llu,%llu\n", v, (unsigned long long)a, (unsigned long long)c);
+    }
+    std::fclose(fp);
+
+    const char* maps[6] = {"heatmap_node_visits.csv", "heatmap_leaf_visits.csv", "heatmap_intersection.csv",
+                           "heatmap_hits_found.csv",  "heatmap_blend_ops.csv",   "heatmap_max_sort_size.csv"};
+    for (int k = 0; k < 6; ++k) {
+        fp = open(maps[k], "w");
+        if (!fp) return set_error(KDGRT_ERR_IO, std::string("stats_write: cannot write ") + maps[k]);
+        for (int y = 0; y < height; ++y) {
+            for (int x = 0; x < width; ++x)
+                std::fprintf(fp, x + 1 < width ? "%d," : "%d", counters[8 * ((size_t)y * width + x) + k]);
+            std::fputc('\n', fp);
+        }
+        std::fclose(fp);
+    }
+
+    // isectCnt_kd.bin and blendCnt_kd.bin: one uint32 per ray; the reference
+    // writes hits_found into both (cudaRenderer.cu:3309-3339)
+    std::vector<uint32_t> hits(np);
+    for (size_t i = 0; i < np; ++i) hits[i] = (uint32_t)counters[8 * i + 3];
+    for (const char* name : {"isectCnt_kd.bin", "blendCnt_kd.bin"}) {
+        fp = open(name, "wb");
+        if (!fp) return set_error(KDGRT_ERR_IO, std::string("stats_write: cannot write ") + name);
+        const bool ok = std::fwrite(hits.data(), 4, np, fp) == np;
+        std::fclose(fp);
+        if (!ok) return set_error(KDGRT_ERR_IO, std::string("stats_write: short write ") + name);
+    }
+    return KDGRT_OK;
+}
+
+} // extern "C"

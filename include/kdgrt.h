@@ -376,6 +376,20 @@ KDGRT_API kdgrt_status kdgrt_ply_read(const char* path, kdgrt_gaussian* out, uin
 /* In-place 1/scale and 1/k_scale (Kd-treeConverterMain.cpp:3636-3642). */
 KDGRT_API kdgrt_status kdgrt_gaussians_invert_scales(kdgrt_gaussian* g, uint64_t n);
 
+/* NeRF transforms.json camera list (loadCameraJson, LoadCamera.hpp:25-62):
+ * per entry of "frames", columns 0/1/2 of "transform_matrix" are the u, v,
+ * n axes and column 3 the position; fovy 39.6 degrees and near 0.005 as in
+ * the reference, aspect from the caller.  Writes min(count, capacity)
+ * cameras and sets *count to the number of frames. */
+KDGRT_API kdgrt_status kdgrt_cameras_from_nerf_json(const char* path, double aspect, kdgrt_camera* out, int capacity,
+                                                    int* count);
+/* Per-ray statistics files of a debug render (the HIT_AND_NODE_COUNT_DEBUG
+ * products, cudaRenderer.cu:3173-3400) into directory `dir`: renderStats.txt,
+ * render_stats.csv, heatmap_{node_visits,leaf_visits,intersection,hits_found,
+ * blend_ops,max_sort_size}.csv, isectCnt_kd.bin, blendCnt_kd.bin.  `counters`
+ * is kdgrt_render_debug's output (8 int32 per ray, ray order y*width + x). */
+KDGRT_API kdgrt_status kdgrt_stats_write(const int32_t* counters, int width, int height, const char* dir);
+
 #ifdef __cplusplus
 }
 #endif
