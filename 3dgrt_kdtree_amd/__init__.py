@@ -14,6 +14,8 @@ Reference entry points mirrored (Kd-treeConverter_v0.9/Kd-treeConverter/):
   Renderer.cleanup()          <- cleanupCudaResources()         cudaRenderer.cu:3486
   build_tree(...)             <- build_kd_tree_for_composite_object  Kd-treeConverter.cpp:22
   Tree.write_kdt / read_kdt   <- dump_/read_kd_tree_...          Kd-treeConverter.cpp:297, 444
+  Device.render_hybrid        <- renderKernelGaussian_Mesh_Hybrid cudaRenderer.cu:2217-2311
+  obj_read_secondary          <- loadSecondaryMesh()             Kd-treeConverterMain.cpp:178-226
 """
 import ctypes as C
 import os
@@ -35,6 +37,17 @@ N_COUNTERS = 8
 COUNTER_NAMES = ["node_visits", "leaf_visits", "isect_tests", "hits_found", "blend_ops", "max_sort", "n_eval",
                  "restarts"]
 
+# Secondary primitives of the hybrid render (SECONDARY_PRIM, Kd-treeConverter.h:28-30)
+# and their materials (MeshHit.materialType, cudaRenderer.cu:2050-2060).
+SECONDARY_TRI = 0
+SECONDARY_SPHERE = 1
+MATERIAL_DIFFUSE = 0
+MATERIAL_MIRROR = 1
+MATERIAL_GLASS = 2
+SECONDARY_SPHERE_DTYPE = np.dtype([("center", "<f4", 3), ("radius", "<f4"), ("material", "<i4"),
+                                   ("reflectivity", "<f4"), ("ior", "<f4"), ("color", "<f4", 3)])
+assert SECONDARY_SPHERE_DTYPE.itemsize == 40
+
 STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "DEVICE", 4: "IO", 5: "FORMAT", 6: "CAPACITY",
           7: "NOT_READY", 8: "NO_DEVICE", 9: "CANCELLED", 10: "TIMEOUT"}
 
@@ -53,6 +66,12 @@ class Camera(C.Structure):
 class CameraInfo(C.Structure):
     _fields_ = [("eye", C.c_float * 3), ("u", C.c_float * 3), ("v", C.c_float * 3), ("start", C.c_float * 3),
                 ("step_x", C.c_float), ("step_y", C.c_float)]
+
+
+class Secondary(C.Structure):
+    _fields_ = [("prim", C.c_int32), ("count", C.c_int32), ("spheres", C.c_void_p), ("d_vertices", C.c_void_p),
+                ("mesh_material", C.c_int32), ("mesh_reflectivity", C.c_float), ("mesh_ior", C.c_float),
+                ("mesh_color", C.c_float * 3), ("max_bounces", C.c_int32)]
 
 
 class BuildParams(C.Structure):
@@ -253,6 +272,24 @@ def cameras_from_nerf_json(path, aspect):
     return list(arr)[:n.value]
 
 
+def demo_spheres():
+    """The reference's two demo spheres (cudaRenderer.cu:2162-2167)."""
+    out = np.zeros(2, SECONDARY_SPHERE_DTYPE)
+    check(lib().kdgrt_secondary_demo_spheres(_p(out)))
+    return out
+
+
+def obj_read_secondary(path):
+    """Triangles of an OBJ file as the hybrid render's secondary mesh,
+    shape (n, 9) float32, vertices stored as (x, -z, y)."""
+    n = C.c_uint64()
+    check(lib().kdgrt_obj_read_secondary(str(path).encode(), None, C.c_uint64(0), C.byref(n)))
+    out = np.zeros((n.value, 9), np.float32)
+    if n.value:
+        check(lib().kdgrt_obj_read_secondary(str(path).encode(), _p(out), C.c_uint64(n.value), C.byref(n)))
+    return out
+
+
 def write_stats(counters, width, height, directory):
     """Writes the reference's per-ray statistics files from render_debug
     counters (shape (height, width, 8) or (height*width, 8), ray order)."""
@@ -349,6 +386,44 @@ class Device:
         img = out.download(np.float32, (h, w, ch))
         out.free()
         return img
+
+    def render_hybrid(self, scene, ci, w, h, spheres=None, triangles=None, mesh_material=MATERIAL_MIRROR,
+                      mesh_reflectivity=0.95, mesh_ior=1.1, mesh_color=(0.95, 0.95, 0.95), max_bounces=5, depth=12,
+                      out=None, stream=None):
+        """Hybrid view (kdgrt_render_hybrid): secondary spheres (an array of
+        SECONDARY_SPHERE_DTYPE) or a triangle mesh ((n, 9) float32, one
+        material).  Returns the (h, w, 3) image, or launches into the device
+        buffer `out` without waiting when one is given."""
+        sec = Secondary()
+        sec.max_bounces = max_bounces
+        keep = []
+        if spheres is not None:
+            sph = np.ascontiguousarray(spheres, SECONDARY_SPHERE_DTYPE)
+            keep.append(sph)
+            sec.prim, sec.count, sec.spheres = SECONDARY_SPHERE, len(sph), sph.ctypes.data
+        else:
+            tri = np.ascontiguousarray(triangles if triangles is not None else np.zeros((0, 9)), np.float32)
+            tri = tri.reshape(-1, 9)
+            dv = self.upload(tri) if len(tri) else None
+            keep.append(dv)
+            sec.prim, sec.count, sec.d_vertices = SECONDARY_TRI, len(tri), dv.ptr if dv else None
+            sec.mesh_material, sec.mesh_reflectivity, sec.mesh_ior = mesh_material, mesh_reflectivity, mesh_ior
+            sec.mesh_color[:] = mesh_color
+        buf = out if out is not None else self.alloc(w * h * 12)
+        try:
+            check(lib().kdgrt_render_hybrid(self.h, scene.h, C.byref(ci), C.c_int(w), C.c_int(h), C.c_int(depth),
+                                            C.byref(sec), buf.ptr, stream))
+            if out is not None:
+                return None
+            self.sync()
+            return buf.download(np.float32, (h, w, 3))
+        finally:
+            if out is None:
+                buf.free()
+            for k in keep:
+                if isinstance(k, DeviceBuffer):
+                    self.sync(stream)
+                    k.free()
 
     def render_debug(self, scene, ci, w, h, depth=12, seq_cap=0):
         out = self.alloc(w * h * 16)

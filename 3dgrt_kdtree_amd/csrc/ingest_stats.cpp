@@ -3,7 +3,9 @@
 //    Kd-treeConverter_v0.9/Kd-treeConverter/LoadCamera.hpp:25-62);
 //  * the per-ray statistics files of HIT_AND_NODE_COUNT_DEBUG renders
 //    (cudaRenderer.cu:3173-3400): renderStats.txt, render_stats.csv,
-//    heatmap_*.csv, isectCnt_kd.bin, blendCnt_kd.bin.
+//    heatmap_*.csv, isectCnt_kd.bin, blendCnt_kd.bin;
+//  * the secondary mesh of the hybrid render from a Wavefront OBJ file
+//    (loadSecondaryMesh, Kd-treeConverterMain.cpp:178-226).
 #include <algorithm>
 #include <cctype>
 #include <cerrno>
@@ -181,9 +183,104 @@ bool read_file(const char* path, std::string& out) {
     return ok;
 }
 
+// One OBJ face vertex reference "v", "v/t", "v//n" or "v/t/n": the position
+// index, 1-based, negative counting back from the last vertex read.
+bool obj_vertex_index(const char* tok, size_t n_vertices, size_t& out) {
+    char* end = nullptr;
+    errno = 0;
+    const long v = std::strtol(tok, &end, 10);
+    if (end == tok || errno || v == 0) return false;
+    const long long idx = v > 0 ? (long long)v - 1 : (long long)n_vertices + v;
+    if (idx < 0 || (size_t)idx >= n_vertices) return false;
+    out = (size_t)idx;
+    return true;
+}
+
 } // namespace
 
 extern "C" {
+
+kdgrt_status kdgrt_obj_read_secondary(const char* path, float* out_xyz, uint64_t capacity, uint64_t* n_triangles) {
+    if (!path || !n_triangles) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "obj_read_secondary: null argument");
+    std::string text;
+    if (!read_file(path, text)) return set_error(KDGRT_ERR_IO, std::string("obj_read_secondary: cannot read ") + path);
+    std::vector<float> vx;         // positions as read, 3 per vertex
+    std::vector<size_t> tri;       // 3 vertex indices per triangle, faces in file order
+    size_t line_no = 0, pos = 0;
+    while (pos < text.size()) {
+        size_t eol = text.find('\n', pos);
+        if (eol == std::string::npos) eol = text.size();
+        std::string line = text.substr(pos, eol - pos);
+        pos = eol + 1;
+        ++line_no;
+        for (char& c : line)
+            if (c == '\r' || c == '\t') c = ' ';
+        size_t b = line.find_first_not_of(' ');
+        if (b == std::string::npos) continue;
+        const char* p = line.c_str() + b;
+        if (p[0] == 'v' && p[1] == ' ') {
+            // "v x y z [w | r g b]": the first three numbers; missing ones are 0
+            float xyz[3] = {0.0f, 0.0f, 0.0f};
+            const char* q = p + 2;
+            for (int k = 0; k < 3; ++k) {
+                char* end = nullptr;
+                const double val = std::strtod(q, &end);
+                if (end == q) break;
+                xyz[k] = (float)val;
+                q = end;
+            }
+            vx.insert(vx.end(), xyz, xyz + 3);
+        } else if (p[0] == 'f' && p[1] == ' ') {
+            std::vector<size_t> face;
+            const char* q = p + 2;
+            while (*q) {
+                while (*q == ' ') ++q;
+                if (!*q) break;
+                const char* tok = q;
+                while (*q && *q != ' ') ++q;
+                size_t idx;
+                if (!obj_vertex_index(tok, vx.size() / 3, idx))
+                    return set_error(KDGRT_ERR_FORMAT, "obj_read_secondary: bad vertex index on line " +
+                                                           std::to_string(line_no));
+                face.push_back(idx);
+            }
+            if (face.size() < 3) continue; // degenerate face, skipped as tinyobjloader does
+            if (face.size() == 3) {
+                tri.insert(tri.end(), face.begin(), face.end());
+            } else if (face.size() == 4) {
+                // quad: split along the shorter diagonal, (0,1,2)(0,2,3) when
+                // |v2 - v0|^2 < |v3 - v1|^2, else (0,1,3)(1,2,3)
+                const float* v0 = &vx[3 * face[0]];
+                const float* v1 = &vx[3 * face[1]];
+                const float* v2 = &vx[3 * face[2]];
+                const float* v3 = &vx[3 * face[3]];
+                const float a0 = v2[0] - v0[0], a1 = v2[1] - v0[1], a2 = v2[2] - v0[2];
+                const float b0 = v3[0] - v1[0], b1 = v3[1] - v1[1], b2 = v3[2] - v1[2];
+                const float s02 = a0 * a0 + a1 * a1 + a2 * a2, s13 = b0 * b0 + b1 * b1 + b2 * b2;
+                const size_t order02[6] = {0, 1, 2, 0, 2, 3}, order13[6] = {0, 1, 3, 1, 2, 3};
+                const size_t* o = s02 < s13 ? order02 : order13;
+                for (int k = 0; k < 6; ++k) tri.push_back(face[o[k]]);
+            } else {
+                return set_error(KDGRT_ERR_FORMAT, "obj_read_secondary: face of " + std::to_string(face.size()) +
+                                                       " vertices on line " + std::to_string(line_no) +
+                                                       " (triangles and quads only)");
+            }
+        }
+    }
+    const uint64_t n = tri.size() / 3;
+    if (out_xyz) {
+        const uint64_t m = std::min<uint64_t>(n, capacity);
+        for (uint64_t i = 0; i < 3 * m; ++i) {
+            // stored as (x, -z, y)
+            const float* v = &vx[3 * tri[i]];
+            out_xyz[3 * i + 0] = v[0];
+            out_xyz[3 * i + 1] = -v[2];
+            out_xyz[3 * i + 2] = v[1];
+        }
+    }
+    *n_triangles = n;
+    return KDGRT_OK;
+}
 
 kdgrt_status kdgrt_cameras_from_nerf_json(const char* path, double aspect, kdgrt_camera* out, int capacity,
                                           int* count) {

@@ -10,6 +10,9 @@
 //              from many views / ray sets / scenes; one queue per XCD with
 //              stealing, each wave pulls its own tile (no block barriers).
 //  k_reset     zeroes the queue counters (graph-safe, no memset node).
+//  k_hybrid    hybrid view (SECONDARY_RAY, cudaRenderer.cu:2217-2311): per
+//              pixel, segments of the Gaussian trace between mirror / glass
+//              secondary primitives, tiled as k_view.
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
@@ -24,6 +27,7 @@
 
 #include "internal.hpp"
 #include "runtime.hpp"
+#include "secondary_device.hpp"
 #include "trace_device.hpp"
 
 using namespace kdgrt_dev;
@@ -161,6 +165,56 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DBG ? 1 : 4)
             for (int k = 0; k < 8; ++k) a.counters[8 * i + k] = dbg.cnt[k];
         if (a.hashes) a.hashes[i] = dbg.hash;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Hybrid view: Gaussians + secondary primitives.
+// ---------------------------------------------------------------------------
+struct HybridArgs {
+    ViewArgs v;
+    SecScene sec;
+};
+
+// renderKernelGaussian_Mesh_Hybrid (cudaRenderer.cu:2217-2311).  Each
+// segment: closest secondary primitive, Gaussian trace up to it (colour and
+// opacity keep accumulating over the segments, as the reference passes
+// them by reference), colour += throughput * accumulated colour, throughput
+// *= 1 - accumulated opacity, stop below 0.01 in every channel, then the
+// material step.  The first segment starts at the eye and uses the view's
+// cone records; later segments use the sphere pre-test.
+template <int K>
+__global__ __launch_bounds__(64) void k_hybrid(HybridArgs ha) {
+    const ViewArgs& a = ha.v;
+    const uint32_t L = xcd_interleave<64>(blockIdx.x);
+    const uint32_t st = L / 64u, local = L % 64u;
+    if (st >= (uint32_t)a.blocks_x * (uint32_t)a.blocks_y) return;
+    const int tx = (int)(st % (uint32_t)a.blocks_x) * 8 + (int)(local % 8u);
+    const int ty = (int)(st / (uint32_t)a.blocks_x) * 8 + (int)(local / 8u);
+    const int lane = threadIdx.x;
+    const int x = tx * 8 + (lane & 7);
+    const int y = ty * 8 + (lane >> 3);
+    if (x >= a.W || y >= a.H) return;
+    float o[3], d[3];
+    make_ray(a.cam, x, y, o, d);
+    float fin[3] = {0.0f, 0.0f, 0.0f}, thr[3] = {1.0f, 1.0f, 1.0f}, acc[3] = {0.0f, 0.0f, 0.0f}, A = 0.0f;
+    extern __shared__ uint2 lds_stack[];
+    for (int b = 0; b < ha.sec.max_bounces; ++b) {
+        float t_hit;
+        const int hit = sec_closest(ha.sec, o, d, t_hit);
+        trace_ray<K, false, false, false, true>(a.sc, o, d, a.depth, lds_stack, lane, acc, A, nullptr,
+                                                hit >= 0 ? t_hit : kSecFar, b == 0);
+        for (int k = 0; k < 3; ++k) fin[k] = fin[k] + thr[k] * acc[k];
+        for (int k = 0; k < 3; ++k) thr[k] = thr[k] * (1.0f - A);
+        if (thr[0] < 0.01f && thr[1] < 0.01f && thr[2] < 0.01f) break;
+        if (hit < 0) break;
+        SecHit h;
+        sec_record(ha.sec, hit, t_hit, o, d, h);
+        if (!sec_bounce(h, b, o, d, fin, thr)) break;
+    }
+    float* f = reinterpret_cast<float*>(a.out) + 3 * ((size_t)(a.H - y - 1) * a.W + x);
+    f[0] = fminf(fin[0], 1.0f);
+    f[1] = fminf(fin[1], 1.0f);
+    f[2] = fminf(fin[2], 1.0f);
 }
 
 // ---------------------------------------------------------------------------
@@ -857,11 +911,13 @@ static void fill_view_args(ViewArgs& a, const kdgrt_scene* sc, const kdgrt_camer
     a.seq_cap = 0;
 }
 
-// Launches one plain view: the cone records of the view's origin into the
-// scene's table, then the view kernel, both on `s`.  Launches of one scene on
-// different streams are ordered through the table's event.
-typedef void (*ViewKernel)(ViewArgs);
-static kdgrt_status launch_view(const kdgrt_scene* csc, ViewArgs& a, ViewKernel kern, hipStream_t s) {
+// Launches one view: the cone records of the view's origin into the scene's
+// table, then `launch` (the view kernel, reading a.sc.vcone), both on `s`.
+// Launches of one scene on different streams are ordered through the
+// table's event.
+extern "C++" {
+template <class Launch>
+static kdgrt_status with_cone_table(const kdgrt_scene* csc, ViewArgs& a, hipStream_t s, Launch&& launch) {
     kdgrt_scene* sc = const_cast<kdgrt_scene*>(csc);
     std::lock_guard<std::mutex> lock(sc->cone_mu);
     if (sc->cone_used && sc->cone_stream != s) HIP_TRY(hipStreamWaitEvent(s, sc->cone_ev, 0));
@@ -870,13 +926,21 @@ static kdgrt_status launch_view(const kdgrt_scene* csc, ViewArgs& a, ViewKernel 
     hipLaunchKernelGGL(k_cone_prep, dim3(prep_grid), dim3(256), 0, s, (const float4*)sc->d_lsph, n, a.cam.eye[0],
                        a.cam.eye[1], a.cam.eye[2], (float4*)sc->d_vcone);
     a.sc.vcone = (const float4*)sc->d_vcone;
-    const dim3 grid(interleave_grid<64>((uint64_t)a.blocks_x * a.blocks_y * 64)), block(64);
-    hipLaunchKernelGGL(kern, grid, block, stack_lds_bytes(a.depth, 64), s, a);
+    launch();
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(sc->cone_ev, s));
     sc->cone_stream = s;
     sc->cone_used = true;
     return KDGRT_OK;
+}
+} // extern "C++"
+
+typedef void (*ViewKernel)(ViewArgs);
+static kdgrt_status launch_view(const kdgrt_scene* csc, ViewArgs& a, ViewKernel kern, hipStream_t s) {
+    return with_cone_table(csc, a, s, [&] {
+        const dim3 grid(interleave_grid<64>((uint64_t)a.blocks_x * a.blocks_y * 64)), block(64);
+        hipLaunchKernelGGL(kern, grid, block, stack_lds_bytes(a.depth, 64), s, a);
+    });
 }
 
 kdgrt_status kdgrt_render(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgrt_camera_info* cam, int W, int H,
@@ -917,6 +981,52 @@ kdgrt_status kdgrt_render_profile(kdgrt_context* ctx, const kdgrt_scene* sc, con
     fill_view_args(a, sc, cam, W, H, 12, out);
     a.prof = phase_cycles; // 8 x u64 per ray
     return launch_view(sc, a, k_view<kViewK, false, KDGRT_OUT_RGBT_F32, true>, (hipStream_t)stream);
+}
+
+kdgrt_status kdgrt_secondary_demo_spheres(kdgrt_secondary_sphere out[2]) {
+    if (!out) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "secondary_demo_spheres: null");
+    // demo_spheres (cudaRenderer.cu:2162-2167)
+    out[0] = kdgrt_secondary_sphere{{0.0f, 1.0f, 0.0f}, 0.3f, KDGRT_MATERIAL_GLASS, 1.0f, 1.1f, {1.0f, 1.0f, 1.0f}};
+    out[1] = kdgrt_secondary_sphere{{0.8f, 1.0f, 1.2f}, 0.3f, KDGRT_MATERIAL_MIRROR, 1.0f, 1.0f, {1.0f, 1.0f, 1.0f}};
+    return KDGRT_OK;
+}
+
+kdgrt_status kdgrt_render_hybrid(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgrt_camera_info* cam, int W,
+                                 int H, int depth, const kdgrt_secondary* sec, float* out, void* stream) {
+    kdgrt_status st = check_view(ctx, sc, cam, W, H, depth, out);
+    if (st != KDGRT_OK) return st;
+    if (!sec) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render_hybrid: null secondary scene");
+    if (sec->max_bounces < 1 || sec->max_bounces > KDGRT_SECONDARY_MAX_BOUNCES)
+        return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render_hybrid: max_bounces must be 1..16");
+    if (sec->count < 0) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render_hybrid: negative primitive count");
+    HybridArgs ha;
+    std::memset(&ha.sec, 0, sizeof(ha.sec));
+    ha.sec.prim = sec->prim;
+    ha.sec.count = sec->count;
+    ha.sec.max_bounces = sec->max_bounces;
+    if (sec->prim == KDGRT_SECONDARY_SPHERE) {
+        if (sec->count > KDGRT_SECONDARY_MAX_SPHERES)
+            return set_error(KDGRT_ERR_CAPACITY, "render_hybrid: at most 8 secondary spheres");
+        if (sec->count && !sec->spheres) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render_hybrid: null spheres");
+        static_assert(sizeof(SecSphere) == sizeof(kdgrt_secondary_sphere), "sphere record layout");
+        if (sec->count) std::memcpy(ha.sec.sph, sec->spheres, sizeof(SecSphere) * (size_t)sec->count);
+    } else if (sec->prim == KDGRT_SECONDARY_TRI) {
+        if (sec->count && !sec->d_vertices)
+            return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render_hybrid: null triangle vertices");
+        ha.sec.tri = sec->d_vertices;
+        ha.sec.mesh_mat = sec->mesh_material;
+        ha.sec.mesh_refl = sec->mesh_reflectivity;
+        ha.sec.mesh_ior = sec->mesh_ior;
+        for (int k = 0; k < 3; ++k) ha.sec.mesh_col[k] = sec->mesh_color[k];
+    } else {
+        return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render_hybrid: prim must be KDGRT_SECONDARY_TRI or _SPHERE");
+    }
+    fill_view_args(ha.v, sc, cam, W, H, depth, out);
+    const hipStream_t s = (hipStream_t)stream;
+    return with_cone_table(sc, ha.v, s, [&] {
+        const dim3 grid(interleave_grid<64>((uint64_t)ha.v.blocks_x * ha.v.blocks_y * 64)), block(64);
+        hipLaunchKernelGGL(k_hybrid<kViewK>, grid, block, stack_lds_bytes(depth, 64), s, ha);
+    });
 }
 
 static kdgrt_status launch_rays(kdgrt_context* ctx, const kdgrt_scene* sc, const float* ro, const float* rd,

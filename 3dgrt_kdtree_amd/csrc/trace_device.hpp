@@ -317,6 +317,8 @@ struct RayState {
     float rgb[3];
     float A;
     int guard;
+    float max_t;    // hybrid segments: the secondary hit that ends the segment
+    uint32_t cone;  // hybrid segments: pre-test with the view's cone records
 };
 
 // Starts a ray: scene-box slab test (cudaRenderer.cu:379-399, 1536-1537),
@@ -419,7 +421,7 @@ __device__ __forceinline__ bool next_leaf(RayState& st, const DevScene& sc, int 
 // and early termination.  The K-entry register buffer holds the leaf's hits
 // by (t, slot); a leaf with more than K hits is finished by further passes
 // that keep only keys above the last composited one.
-template <int K, bool DBG, bool PROF, bool CONE>
+template <int K, bool DBG, bool PROF, bool CONE, bool HYB>
 __device__ __forceinline__ void leaf(RayState& st, const DevScene& sc, RayDbg* dbg) {
     const uint32_t base = st.node.y, count = st.node.x >> 3;
     if (DBG) {
@@ -456,7 +458,7 @@ __device__ __forceinline__ void leaf(RayState& st, const DevScene& sc, RayDbg* d
             // are cleared afterwards
             for (uint32_t j0 = 0; j0 < wn; j0 += 4) {
                 uint32_t bits;
-                if (CONE) {
+                if (CONE || (HYB && st.cone)) {
                     const float4* cp = sc.vcone + base + w0 + j0;
                     const float4 c0 = cp[0], c1 = cp[1], c2 = cp[2], c3 = cp[3];
                     bits = (cone_pass(d, c0) ? 1u : 0u) | (cone_pass(d, c1) ? 2u : 0u) |
@@ -509,6 +511,11 @@ __device__ __forceinline__ void leaf(RayState& st, const DevScene& sc, RayDbg* d
 #pragma unroll
         for (int i = 0; i < K; ++i) {
             if (i >= n || done) break;
+            // hybrid segments stop at the secondary hit (cudaRenderer.cu:1612-1616)
+            if (HYB && kt[i] > st.max_t) {
+                done = true;
+                break;
+            }
             if (DBG) dbg->cnt[6]++;
             // particleResponse<4> and the skip rule (cudaRenderer.cu:1655-1671)
             const float gd = kg[i];
@@ -558,17 +565,38 @@ __host__ __device__ constexpr size_t wave_lds_bytes(int depth) { return (size_t)
 // Whole ray: outer loop of cudaRenderer.cu:1552-1709.  CONE: the ray starts
 // at the view origin that sc.vcone was prepared for.  wave_lds: this wave's
 // LDS (wave_lds_bytes(depth)); lane: the lane index within the wave.
-template <int K, bool DBG, bool PROF = false, bool CONE = false>
+// HYB: one segment of a hybrid path (SECONDARY_RAY, cudaRenderer.cu:1538-1541,
+// 1612-1616): rgb and A come in as the path's running colour and opacity and
+// are accumulated further (the reference passes them by reference and does
+// not reset them between segments), t_scene_far is clamped to max_t, and
+// cone_rt selects the cone pre-test at run time (the first segment of a view
+// ray) while later segments use the sphere pre-test.
+template <int K, bool DBG, bool PROF = false, bool CONE = false, bool HYB = false>
 __device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], const float d[3], int depth,
                                           uint2* __restrict__ wave_lds, int lane, float rgb[3], float& A,
-                                          RayDbg* dbg) {
+                                          RayDbg* dbg, float max_t = 0.0f, bool cone_rt = false) {
     uint32_t* const stk = reinterpret_cast<uint32_t*>(wave_lds) + lane;
     RayState st;
-    if (ray_begin(st, sc, o, d, depth)) {
+    bool go = ray_begin(st, sc, o, d, depth);
+    if (HYB) {
+        st.rgb[0] = rgb[0];
+        st.rgb[1] = rgb[1];
+        st.rgb[2] = rgb[2];
+        st.A = A;
+        st.max_t = max_t;
+        st.cone = cone_rt ? 1u : 0u;
+        if (go) {
+            const float tf = fminf(st.t_scene_far, max_t);
+            st.t_scene_far = tf;
+            st.t_far = tf;
+            go = !(st.t_near > tf) && (st.A < 0.99f);
+        }
+    }
+    if (go) {
         while (true) {
             descend<DBG>(st, sc, depth, stk, dbg);
             KD_PROF_MARK(0);
-            leaf<K, DBG, PROF, CONE>(st, sc, dbg);
+            leaf<K, DBG, PROF, CONE, HYB>(st, sc, dbg);
             const bool more = next_leaf<DBG>(st, sc, depth, stk, dbg);
             KD_PROF_MARK(4);
             if (!more) break;

@@ -287,6 +287,40 @@ def serve_bench(dev, scenes, aabbs, counts, tuning, rates=((200, 4.0), (1000, 2.
     return out
 
 
+def hybrid_spheres(aabb):
+    """Secondary spheres of the hybrid leg, sized to the scene box: glass
+    (ior 1.3), tinted mirror (reflectivity 0.85) and diffuse, around the
+    scene centre."""
+    lo, hi = np.asarray(aabb[:3], np.float64), np.asarray(aabb[3:], np.float64)
+    c, ext = 0.5 * (lo + hi), float((hi - lo).max())
+    a = np.zeros(3, K.SECONDARY_SPHERE_DTYPE)
+    a[0] = (c + 0.10 * ext * np.array([1.0, 0.3, 0.2]), 0.08 * ext, K.MATERIAL_GLASS, 1.0, 1.3, (0.95, 0.9, 0.85))
+    a[1] = (c - 0.12 * ext * np.array([1.0, -0.2, 0.1]), 0.10 * ext, K.MATERIAL_MIRROR, 0.85, 1.0, (0.6, 0.7, 0.9))
+    a[2] = (c + 0.20 * ext * np.array([0.0, 1.0, 0.0]), 0.04 * ext, K.MATERIAL_DIFFUSE, 0.0, 1.0, (0.8, 0.2, 0.1))
+    return a
+
+
+def hybrid_bench(dev, sc, ci, aabb, W, H, stream, steps, warmup):
+    """The hybrid view (k_hybrid) of the same scene and camera with three
+    secondary spheres in view, timed like the plain view."""
+    sph = hybrid_spheres(aabb)
+    buf = dev.alloc(W * H * 12)
+    for _ in range(warmup):
+        dev.render_hybrid(sc, ci, W, H, spheres=sph, out=buf, stream=stream)
+    dev.sync()
+    dev.timer_begin(stream)
+    for _ in range(steps):
+        dev.render_hybrid(sc, ci, W, H, spheres=sph, out=buf, stream=stream)
+    ms = dev.timer_end(stream) / steps
+    dev.sync()
+    buf.free()
+    r = {"workload": "hybrid view: same scene and 1920x1080 view, 3 secondary spheres (glass, mirror, diffuse) "
+                     "around the scene centre, up to 5 segments per pixel",
+         "ms_per_launch": round(ms, 4), "primary_mrays_s": round(W * H / (ms * 1e-3) / 1e6, 2)}
+    log(f"hybrid 1080p: {ms:.3f} ms/launch, {r['primary_mrays_s']} primary Mrays/s")
+    return r
+
+
 def cpu_baseline(tree, g_inv, ci, w, h, budget_s):
     import oracle_lib as O
     threads = O.lib().oracle_max_threads()
@@ -313,6 +347,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-multi", action="store_true", help="skip the multi-scene batch (256 views + 4M rays)")
+    ap.add_argument("--no-hybrid", action="store_true", help="skip the hybrid (secondary-sphere) view")
     ap.add_argument("--serve-scenes", type=int, default=16)
     ap.add_argument("--serve-window-ms", type=float, default=0.0,
                     help="how long a busy device waits to fill a batch (0: launch as soon as a slot is free)")
@@ -394,6 +429,9 @@ def main():
               **({"phases": phases} if phases else {}),
               "build": {"host_sah_build_s": round(build_s, 2), "threads": os.cpu_count(), "nodes": st["n_nodes"],
                         "leaf_refs": st["n_offsets"], "proxy_triangles": st["n_triangles"]}}
+
+    if not args.no_hybrid:
+        result["hybrid"] = hybrid_bench(dev, sc, ci, aabb, W, H, stream, args.steps, args.warmup)
 
     if not args.no_batched:
         views = views_64(aabb)

@@ -28,6 +28,9 @@
  *   kdgrt_kdt_write/read   <- dump_kd_tree_for_composite_object / read_kd_tree_from_file
  *                                                              Kd-treeConverter.cpp:297-343, 444-492
  *   kdgrt_ply_read         <- loadGaussiansFromPly()           Kd-treeConverterMain.cpp:1652-1769
+ *   kdgrt_render_hybrid    <- renderGaussianWithCudaFrame() with SECONDARY_RAY
+ *                             (kernel renderKernelGaussian_Mesh_Hybrid, :2217-2311)
+ *   kdgrt_obj_read_secondary <- loadSecondaryMesh()            Kd-treeConverterMain.cpp:178-226
  *   kdgrt_render_rays, kdgrt_batch_*, kdgrt_server_* have no single
  *   reference counterpart: they run the same per-ray path over explicit ray
  *   lists, many views in one persistent launch, and a serving loop.
@@ -232,6 +235,58 @@ KDGRT_API kdgrt_status kdgrt_render_rays_debug(kdgrt_context* ctx, const kdgrt_s
                                                uint64_t* id_hash, void* stream);
 KDGRT_API kdgrt_status kdgrt_warm_up(kdgrt_context* ctx, const kdgrt_scene* scene, void* stream);
 
+/* ---------------- hybrid render: Gaussians + reflective/refractive secondary primitives ---------------- */
+/* Secondary primitive kinds, the values of SECONDARY_PRIM (Kd-treeConverter.h:28-30). */
+#define KDGRT_SECONDARY_TRI 0
+#define KDGRT_SECONDARY_SPHERE 1
+/* Material types of MeshHit (cudaRenderer.cu:2050-2060). */
+#define KDGRT_MATERIAL_DIFFUSE 0 /* adds its colour and ends the path */
+#define KDGRT_MATERIAL_MIRROR 1  /* reflection */
+#define KDGRT_MATERIAL_GLASS 2   /* refraction (total internal reflection when sinT^2 > 1) */
+#define KDGRT_SECONDARY_MAX_SPHERES 8
+#define KDGRT_SECONDARY_MAX_BOUNCES 16
+
+/* DemoSphere (cudaRenderer.cu:2153-2160). */
+typedef struct kdgrt_secondary_sphere {
+    float center[3];
+    float radius;
+    int32_t material;
+    float reflectivity;
+    float ior;
+    float color[3];
+} kdgrt_secondary_sphere;
+
+/* The secondary scene of one hybrid launch.  prim KDGRT_SECONDARY_SPHERE:
+ * `count` spheres from the host array `spheres` (copied at launch; the
+ * reference's constant pair is kdgrt_secondary_demo_spheres).  prim
+ * KDGRT_SECONDARY_TRI: `count` triangles, 9 floats each, in device memory
+ * `d_vertices`, all with the material of the mesh fields (SecondaryMesh,
+ * cudaRenderer.h:11-19).  max_bounces: MAX_BOUNCES (5 in the reference). */
+typedef struct kdgrt_secondary {
+    int32_t prim;
+    int32_t count;
+    const kdgrt_secondary_sphere* spheres;
+    const float* d_vertices;
+    int32_t mesh_material;
+    float mesh_reflectivity;
+    float mesh_ior;
+    float mesh_color[3];
+    int32_t max_bounces;
+} kdgrt_secondary;
+
+/* The reference's two demo spheres (cudaRenderer.cu:2162-2167): a glass
+ * sphere at (0, 1, 0) and a mirror sphere at (0.8, 1, 1.2), radius 0.3. */
+KDGRT_API kdgrt_status kdgrt_secondary_demo_spheres(kdgrt_secondary_sphere out[2]);
+/* One view through renderKernelGaussian_Mesh_Hybrid (cudaRenderer.cu:2217-2311):
+ * per pixel, up to max_bounces segments; each segment intersects the
+ * secondary primitives first, traces the Gaussians up to that hit
+ * (t_scene_far clamped to it, cudaRenderer.cu:1538-1541, 1612-1616) and
+ * follows the mirror / glass bounce.  out_rgb: width*height float3,
+ * min(colour, 1) per channel, rows flipped as in kdgrt_render. */
+KDGRT_API kdgrt_status kdgrt_render_hybrid(kdgrt_context* ctx, const kdgrt_scene* scene,
+                                           const kdgrt_camera_info* cam, int width, int height, int stack_depth,
+                                           const kdgrt_secondary* secondary, float* out_rgb, void* stream);
+
 /* ---------------- batched persistent launch ---------------- */
 typedef struct kdgrt_view {
     const kdgrt_scene* scene;
@@ -383,6 +438,14 @@ KDGRT_API kdgrt_status kdgrt_gaussians_invert_scales(kdgrt_gaussian* g, uint64_t
  * cameras and sets *count to the number of frames. */
 KDGRT_API kdgrt_status kdgrt_cameras_from_nerf_json(const char* path, double aspect, kdgrt_camera* out, int capacity,
                                                     int* count);
+/* Triangles of a Wavefront OBJ file as the secondary mesh (loadSecondaryMesh,
+ * Kd-treeConverterMain.cpp:178-226): faces in file order, quads split along
+ * the shorter diagonal as tinyobjloader's triangulation does, each vertex
+ * stored as (x, -z, y).  Faces of more than four vertices are rejected
+ * (KDGRT_ERR_FORMAT).  Writes min(n, capacity) triangles (9 floats each)
+ * and sets *n_triangles; pass out_xyz = NULL to count. */
+KDGRT_API kdgrt_status kdgrt_obj_read_secondary(const char* path, float* out_xyz, uint64_t capacity,
+                                                uint64_t* n_triangles);
 /* Per-ray statistics files of a debug render (the HIT_AND_NODE_COUNT_DEBUG
  * products, cudaRenderer.cu:3173-3400) into directory `dir`: renderStats.txt,
  * render_stats.csv, heatmap_{node_visits,leaf_visits,intersection,hits_found,

@@ -25,10 +25,16 @@
 // Parity against NVIDIA FP32 bits is therefore "parity unpinned"; this
 // restatement is the definitional FP32 result.
 //
+// and of the hybrid render (renderKernelGaussian_Mesh_Hybrid, SECONDARY_RAY=1,
+// cudaRenderer.cu:2217-2311) with sphere or triangle secondary primitives.
+//
 // Pins (tests/): closed forms for ray generation, slab test, intersection,
 // response and SH; brute force on single-leaf scenes; hand-built trees with
 // hand-counted node visits (ring-stack overwrite and restart); FP64
-// agreement; golden fixtures.
+// agreement; golden fixtures.  Hybrid: closed-form sphere and triangle hits,
+// mirror / Snell / total-internal-reflection directions computed in double,
+// the running-colour sum of a mirror path by hand, segments without a
+// secondary hit equal to the plain path.
 
 #include <cfloat>
 #include <cmath>
@@ -197,13 +203,20 @@ inline void ray_prim(const V3<T>& o, const V3<T>& d, uint32_t id, const Gauss256
 // branch (cudaRenderer.cu:1488-1752), with the HIT_AND_NODE_COUNT_DEBUG
 // counters (:1504-1606, 1679-1681).  D = short-stack depth (ring of
 // cudaRenderer.cu:189-210; the reference uses SHORT_STACK_DEPTH 12).
+// hybrid: one segment of the hybrid path (SECONDARY_RAY, :1500-1502,
+// 1538-1541, 1612-1616): color and A are the running values passed in by
+// reference and not reset, t_scene_far is clamped to max_t, and the composite
+// loop stops at a hit beyond max_t.
 template <class T>
-void trace(const Scene& sc, V3<T> o, V3<T> d, int D, V3<T>& color, T& A, RayOut& ro, uint32_t* seq, int seq_cap) {
+void trace(const Scene& sc, V3<T> o, V3<T> d, int D, V3<T>& color, T& A, RayOut& ro, uint32_t* seq, int seq_cap,
+           bool hybrid = false, T max_t = T(0)) {
     std::memset(ro.cnt, 0, sizeof(ro.cnt));
     ro.hash = kFnvBasis;
     int nseq = 0;
-    color = {T(0), T(0), T(0)};
-    A = T(0);
+    if (!hybrid) {
+        color = {T(0), T(0), T(0)};
+        A = T(0);
+    }
     // BoundsRayIntersect (cudaRenderer.cu:379-399), t in [1e-2, FLT_MAX].
     T t_scene_near = T(1e-2f), t_scene_far = T(FLT_MAX);
     {
@@ -216,6 +229,10 @@ void trace(const Scene& sc, V3<T> o, V3<T> d, int D, V3<T>& color, T& A, RayOut&
             t_scene_far = tmin(tmax(l1, l2), t_scene_far);
         }
         if (!((t_scene_far >= t_scene_near) & (t_scene_far >= T(0)))) return;
+    }
+    if (hybrid) {
+        t_scene_far = tmin(t_scene_far, max_t);
+        if (t_scene_near > t_scene_far) return;
     }
     // Ring stack (cudaRenderer.cu:189-210).
     std::vector<uint32_t> st_node(D);
@@ -269,6 +286,7 @@ void trace(const Scene& sc, V3<T> o, V3<T> d, int D, V3<T>& color, T& A, RayOut&
         }
         // Composite loop (cudaRenderer.cu:1608-1688).
         for (size_t i = 0; i < hits.size(); ++i) {
+            if (hybrid && hits[i].t > max_t) break;
             ro.cnt[6]++; // N_eval
             const uint32_t gid = hits[i].id;
             const Gauss256& g = sc.g[gid];
@@ -379,6 +397,197 @@ void render_rays(const Scene& sc, const float* origins, const float* dirs, uint6
     }
 }
 
+// ---- hybrid render (renderKernelGaussian_Mesh_Hybrid, cudaRenderer.cu:2217-2311) ----
+
+struct SecSphere { // DemoSphere (cudaRenderer.cu:2153-2160); same layout as kdgrt_secondary_sphere
+    float c[3];
+    float r;
+    int32_t mat;
+    float refl, ior;
+    float col[3];
+};
+
+struct SecScene {
+    int prim; // 1 spheres, 0 triangles (SECONDARY_PRIM, Kd-treeConverter.h:28-30)
+    int count;
+    const SecSphere* sph;
+    const float* tri; // 9 floats per triangle
+    int32_t mesh_mat;
+    float mesh_refl, mesh_ior, mesh_col[3];
+    int max_bounces; // MAX_BOUNCES
+};
+
+template <class T>
+struct MeshHit { // cudaRenderer.cu:2050-2060
+    bool hit = false;
+    T t = T(1e8f);
+    V3<T> normal{}, hitPoint{};
+    int mat = 0;
+    T refl = T(0), ior = T(0);
+    V3<T> color{};
+    bool front = true;
+};
+
+const float kEps = 0.00001f; // EPSILON (DefineConst.h:24)
+
+// rayTriIntersect (cudaRenderer.cu:1017-1065).
+template <class T>
+bool ray_tri(V3<T> o, V3<T> d, V3<T> v0, V3<T> v1, V3<T> v2, T& t_out) {
+    const V3<T> e1 = sub(v1, v0), e2 = sub(v2, v0);
+    const V3<T> pvec = cross(d, e2);
+    const T det = dot(e1, pvec);
+    if (det > T(-kEps) && det < T(kEps)) return false;
+    const T inv_det = T(1) / det;
+    const V3<T> tvec = sub(o, v0);
+    const T u = dot(tvec, pvec) * inv_det;
+    if (u < T(0) || u > T(1)) return false;
+    const V3<T> qvec = cross(tvec, e1);
+    const T v = dot(d, qvec) * inv_det;
+    if (v < T(0) || u + v > T(1)) return false;
+    const T t = dot(e2, qvec) * inv_det;
+    if (t > T(kEps)) {
+        t_out = t;
+        return true;
+    }
+    return false;
+}
+
+// intersectSphere (cudaRenderer.cu:2170-2215) / intersectTriangleMesh (:2111-2151).
+template <class T>
+MeshHit<T> sec_intersect(const SecScene& s, V3<T> o, V3<T> d) {
+    MeshHit<T> h;
+    if (s.prim == 1) {
+        for (int i = 0; i < s.count; ++i) {
+            const SecSphere& q = s.sph[i];
+            const V3<T> c = {T(q.c[0]), T(q.c[1]), T(q.c[2])};
+            const V3<T> oc = sub(o, c);
+            const T a = dot(d, d);
+            const T b = T(2) * dot(oc, d);
+            const T cc = dot(oc, oc) - T(q.r) * T(q.r);
+            const T disc = b * b - T(4) * a * cc;
+            if (disc >= T(0)) {
+                const T sq = tsqrt(disc);
+                T root = (-b - sq) / (T(2) * a);
+                if (root < T(kEps)) root = (-b + sq) / (T(2) * a);
+                if (root > T(kEps) && root < h.t) {
+                    h.hit = true;
+                    h.t = root;
+                    h.hitPoint = add(o, muls(d, h.t));
+                    h.normal = normalize(sub(h.hitPoint, c));
+                    if (dot(d, h.normal) > T(0)) {
+                        h.normal = muls(h.normal, T(-1));
+                        h.front = false;
+                    } else {
+                        h.front = true;
+                    }
+                    h.mat = q.mat;
+                    h.refl = T(q.refl);
+                    h.ior = T(q.ior);
+                    h.color = {T(q.col[0]), T(q.col[1]), T(q.col[2])};
+                }
+            }
+        }
+    } else {
+        for (int i = 0; i < s.count; ++i) {
+            const float* v = s.tri + 9 * (size_t)i;
+            const V3<T> v0 = {T(v[0]), T(v[1]), T(v[2])}, v1 = {T(v[3]), T(v[4]), T(v[5])},
+                        v2 = {T(v[6]), T(v[7]), T(v[8])};
+            T t;
+            if (ray_tri(o, d, v0, v1, v2, t) && t < h.t) {
+                h.hit = true;
+                h.t = t;
+                h.hitPoint = add(o, muls(d, t));
+                h.normal = normalize(cross(sub(v1, v0), sub(v2, v0)));
+                if (dot(d, h.normal) > T(0)) {
+                    h.normal = muls(h.normal, T(-1));
+                    h.front = false;
+                } else {
+                    h.front = true;
+                }
+                h.mat = s.mesh_mat;
+                h.refl = T(s.mesh_refl);
+                h.ior = T(s.mesh_ior);
+                h.color = {T(s.mesh_col[0]), T(s.mesh_col[1]), T(s.mesh_col[2])};
+            }
+        }
+    }
+    return h;
+}
+
+// The per-pixel bounce loop (cudaRenderer.cu:2228-2304); returns the
+// unclamped colour and the number of segments traced.  path (optional):
+// per segment origin, direction and max_t (7 values).
+template <class T>
+V3<T> hybrid_pixel(const Scene& sc, const SecScene& s, V3<T> o, V3<T> d, int D, int& segments,
+                   T* path = nullptr) {
+    V3<T> fin = {T(0), T(0), T(0)}, thr = {T(1), T(1), T(1)}, acc = {T(0), T(0), T(0)};
+    T A = T(0);
+    segments = 0;
+    for (int bounce = 0; bounce < s.max_bounces; ++bounce) {
+        const MeshHit<T> mh = sec_intersect<T>(s, o, d);
+        const T max_t = mh.hit ? mh.t : T(1e8f);
+        if (path) {
+            const T rec[7] = {o.x, o.y, o.z, d.x, d.y, d.z, max_t};
+            std::memcpy(path + 7 * segments, rec, sizeof(rec));
+        }
+        RayOut ro;
+        trace<T>(sc, o, d, D, acc, A, ro, nullptr, 0, true, max_t);
+        ++segments;
+        fin = add(fin, mul(thr, acc));
+        thr = muls(thr, T(1) - A);
+        if (thr.x < T(0.01f) && thr.y < T(0.01f) && thr.z < T(0.01f)) break;
+        if (!mh.hit) break;
+        const V3<T> N = mh.normal, I = d;
+        if (mh.mat == 1) { // reflection
+            const V3<T> rd = sub(I, muls(muls(N, T(2)), dot(I, N)));
+            o = add(mh.hitPoint, muls(N, T(1e-4f)));
+            d = normalize(rd);
+            V3<T> mc = mh.color;
+            if (bounce > 0) mc = {T(0), T(0), T(0)};
+            fin = add(fin, mul(muls(thr, T(1) - mh.refl), mc));
+            thr = muls(thr, mh.refl);
+        } else if (mh.mat == 2) { // refraction
+            const T eta = mh.front ? T(1) / mh.ior : mh.ior;
+            const T cosI = -dot(N, I);
+            const T sinT2 = eta * eta * (T(1) - cosI * cosI);
+            if (sinT2 > T(1)) { // total internal reflection
+                const V3<T> rd = sub(I, muls(muls(N, T(2)), dot(I, N)));
+                o = add(mh.hitPoint, muls(N, T(1e-4f)));
+                d = normalize(rd);
+            } else {
+                const T cosT = tsqrt(T(1) - sinT2);
+                const V3<T> rd = add(muls(I, eta), muls(N, eta * cosI - cosT));
+                o = sub(mh.hitPoint, muls(N, T(1e-4f)));
+                d = normalize(rd);
+            }
+            thr = mul(thr, mh.color);
+        } else {
+            fin = add(fin, mul(thr, mh.color));
+            break;
+        }
+    }
+    return fin;
+}
+
+template <class T>
+void render_hybrid(const Scene& sc, const SecScene& s, const CamInfo& cam, int W, int H, int D, T* out_rgb,
+                   int32_t* segments, int threads) {
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic)
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            V3<T> o, d;
+            make_ray<T>(cam, x, y, o, d);
+            int nseg = 0;
+            const V3<T> c = hybrid_pixel<T>(sc, s, o, d, D, nseg);
+            const size_t pid = (size_t)(H - y - 1) * W + x;
+            out_rgb[3 * pid + 0] = tmin(c.x, T(1));
+            out_rgb[3 * pid + 1] = tmin(c.y, T(1));
+            out_rgb[3 * pid + 2] = tmin(c.z, T(1));
+            if (segments) segments[(size_t)y * W + x] = nseg;
+        }
+}
+
 Scene make_scene(const void* nodes, const uint32_t* offsets, const void* g, const float* aabb) {
     Scene sc;
     sc.nodes = (const Node*)nodes;
@@ -423,6 +632,75 @@ void oracle_render_rays_f64(const void* nodes, const uint32_t* offsets, const vo
                         threads);
 }
 float oracle_expf(float x) { return kd_expf(x); }
+
+// Hybrid render.  prim 1: `count` spheres (40-B records: centre, radius,
+// material, reflectivity, ior, colour); prim 0: `count` triangles (9 floats)
+// with the mesh material `mesh` = {material, reflectivity, ior, r, g, b}.
+// out_rgb: W*H*3 (row-flipped, min(c, 1)); segments (optional): traced
+// segments per ray by y*W+x.
+static SecScene make_sec(int prim, int count, const void* spheres, const float* tri, const float* mesh,
+                         int max_bounces) {
+    SecScene s;
+    s.prim = prim;
+    s.count = count;
+    s.sph = (const SecSphere*)spheres;
+    s.tri = tri;
+    s.mesh_mat = mesh ? (int32_t)mesh[0] : 0;
+    s.mesh_refl = mesh ? mesh[1] : 0.f;
+    s.mesh_ior = mesh ? mesh[2] : 0.f;
+    for (int k = 0; k < 3; ++k) s.mesh_col[k] = mesh ? mesh[3 + k] : 0.f;
+    s.max_bounces = max_bounces;
+    return s;
+}
+void oracle_render_hybrid_f32(const void* nodes, const uint32_t* offsets, const void* g, const float* aabb,
+                              const void* cam, int W, int H, int D, int prim, int count, const void* spheres,
+                              const float* tri, const float* mesh, int max_bounces, float* out_rgb,
+                              int32_t* segments, int threads) {
+    render_hybrid<float>(make_scene(nodes, offsets, g, aabb), make_sec(prim, count, spheres, tri, mesh, max_bounces),
+                         *(const CamInfo*)cam, W, H, D, out_rgb, segments, threads);
+}
+void oracle_render_hybrid_f64(const void* nodes, const uint32_t* offsets, const void* g, const float* aabb,
+                              const void* cam, int W, int H, int D, int prim, int count, const void* spheres,
+                              const float* tri, const float* mesh, int max_bounces, double* out_rgb,
+                              int32_t* segments, int threads) {
+    render_hybrid<double>(make_scene(nodes, offsets, g, aabb),
+                          make_sec(prim, count, spheres, tri, mesh, max_bounces), *(const CamInfo*)cam, W, H, D,
+                          out_rgb, segments, threads);
+}
+// Segments of one hybrid path through the secondary primitives alone (no
+// Gaussians): up to max_bounces records of origin, direction and max_t
+// (7 floats each); returns the number of segments and writes the colour.
+int oracle_hybrid_path_f32(int prim, int count, const void* spheres, const float* tri, const float* mesh,
+                           int max_bounces, const float* o, const float* d, float* path, float* rgb) {
+    const SecScene s = make_sec(prim, count, spheres, tri, mesh, max_bounces);
+    // a scene box the rays never enter: the traces add nothing
+    static const Node empty_leaf = {3u, 0u};
+    static const uint32_t no_offsets[1] = {0};
+    Scene sc;
+    sc.nodes = &empty_leaf;
+    sc.offsets = no_offsets;
+    sc.g = nullptr;
+    for (int k = 0; k < 3; ++k) {
+        sc.bmin[k] = 1e30f;
+        sc.bmax[k] = 1e30f;
+    }
+    int nseg = 0;
+    const V3<float> c = hybrid_pixel<float>(sc, s, {o[0], o[1], o[2]}, {d[0], d[1], d[2]}, 12, nseg, path);
+    rgb[0] = c.x;
+    rgb[1] = c.y;
+    rgb[2] = c.z;
+    return nseg;
+}
+// One secondary intersection (tests of the closed forms): writes
+// {hit, t, normal xyz, hit point xyz, front} as floats.
+void oracle_sec_intersect_f32(int prim, int count, const void* spheres, const float* tri, const float* o,
+                              const float* d, float* out9) {
+    const SecScene s = make_sec(prim, count, spheres, tri, nullptr, 1);
+    const MeshHit<float> h = sec_intersect<float>(s, {o[0], o[1], o[2]}, {d[0], d[1], d[2]});
+    const float r[9] = {h.hit ? 1.f : 0.f, h.t, h.normal.x, h.normal.y, h.normal.z, h.hitPoint.x, h.hitPoint.y,
+                        h.hitPoint.z, h.front ? 1.f : 0.f};
+    std::memcpy(out9, r, sizeof(r));
+}
 
 // Ray directions of a view (for tests of D1/D2).
 void oracle_ray_dirs(const void* cam, int W, int H, float* dirs) {

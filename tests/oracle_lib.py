@@ -91,3 +91,67 @@ def ray_dirs(ci, w, h):
 
 def expf(x):
     return lib().oracle_expf(float(x))
+
+
+def render_hybrid(nodes, offsets, g_inv, aabb, ci, w, h, spheres=None, triangles=None, mesh=(1, 0.95, 1.1, 0.95, 0.95,
+                  0.95), max_bounces=5, depth=12, f64=False, threads=0):
+    """Hybrid render (renderKernelGaussian_Mesh_Hybrid).  spheres: array of
+    the 40-B sphere records; triangles: (n, 9) float32 with the mesh
+    material (type, reflectivity, ior, r, g, b).  Returns rgb (h, w, 3),
+    row-flipped and clamped to 1 like the device output, and the number of
+    segments traced per ray (h, w)."""
+    nodes = np.ascontiguousarray(nodes)
+    offsets = np.ascontiguousarray(offsets, np.uint32)
+    if len(offsets) == 0:
+        offsets = np.zeros(1, np.uint32)
+    aabb = np.ascontiguousarray(aabb, np.float32)
+    cam = np.frombuffer(bytes(ci), dtype=np.uint8).copy()
+    out = np.zeros((h, w, 3), np.float64 if f64 else np.float32)
+    seg = np.zeros((h, w), np.int32)
+    if spheres is not None:
+        sph = np.ascontiguousarray(spheres)
+        assert sph.dtype.itemsize == 40
+        prim, count, tri, mesh_arr = 1, len(sph), None, None
+    else:
+        sph = None
+        tri = np.ascontiguousarray(triangles, np.float32).reshape(-1, 9)
+        prim, count = 0, len(tri)
+        mesh_arr = np.asarray(mesh, np.float32)
+    fn = lib().oracle_render_hybrid_f64 if f64 else lib().oracle_render_hybrid_f32
+    fn(_p(nodes), _p(offsets), _p(g_inv), _p(aabb), _p(cam), C.c_int(w), C.c_int(h), C.c_int(depth), C.c_int(prim),
+       C.c_int(count), _p(sph), _p(tri), _p(mesh_arr), C.c_int(max_bounces), _p(out), _p(seg), C.c_int(threads))
+    return {"rgb": out, "segments": seg}
+
+
+def sec_intersect(origin, direction, spheres=None, triangles=None):
+    """One secondary intersection: (hit, t, normal, hit point, front)."""
+    o = np.ascontiguousarray(origin, np.float32)
+    d = np.ascontiguousarray(direction, np.float32)
+    out = np.zeros(9, np.float32)
+    if spheres is not None:
+        sph = np.ascontiguousarray(spheres)
+        lib().oracle_sec_intersect_f32(C.c_int(1), C.c_int(len(sph)), _p(sph), None, _p(o), _p(d), _p(out))
+    else:
+        tri = np.ascontiguousarray(triangles, np.float32).reshape(-1, 9)
+        lib().oracle_sec_intersect_f32(C.c_int(0), C.c_int(len(tri)), None, _p(tri), _p(o), _p(d), _p(out))
+    return bool(out[0]), float(out[1]), out[2:5].copy(), out[5:8].copy(), bool(out[8])
+
+
+def hybrid_path(origin, direction, spheres=None, triangles=None, mesh=(1, 0.95, 1.1, 0.95, 0.95, 0.95),
+                max_bounces=5):
+    """Segments of one hybrid path through the secondary primitives alone:
+    (segments (n, 7) = origin, direction, max_t; colour (3,))."""
+    o = np.ascontiguousarray(origin, np.float32)
+    d = np.ascontiguousarray(direction, np.float32)
+    path = np.zeros((max_bounces, 7), np.float32)
+    rgb = np.zeros(3, np.float32)
+    if spheres is not None:
+        sph = np.ascontiguousarray(spheres)
+        n = lib().oracle_hybrid_path_f32(C.c_int(1), C.c_int(len(sph)), _p(sph), None, None, C.c_int(max_bounces),
+                                         _p(o), _p(d), _p(path), _p(rgb))
+    else:
+        tri = np.ascontiguousarray(triangles, np.float32).reshape(-1, 9)
+        m = np.asarray(mesh, np.float32)
+        n = lib().oracle_hybrid_path_f32(C.c_int(0), C.c_int(len(tri)), None, _p(tri), _p(m), C.c_int(max_bounces),
+                                         _p(o), _p(d), _p(path), _p(rgb))
+    return path[:n], rgb

@@ -81,3 +81,30 @@ def single_leaf_scene(n=12, seed=7, opacity=0.3):
     g["f_dc"][:] = rng.uniform(-1, 1, (n, 3))
     g["f_rest"][:] = rng.normal(0, 0.2, (n, 45))
     return HostScene(g, K.Tree.build(g))
+
+
+def spheres_in_view(aabb):
+    """Three secondary spheres around the scene centre, sized to the box: a
+    glass sphere (ior 1.3), a tinted mirror (reflectivity 0.85) and a
+    diffuse one."""
+    lo, hi = np.asarray(aabb[:3], np.float64), np.asarray(aabb[3:], np.float64)
+    c, ext = 0.5 * (lo + hi), float((hi - lo).max())
+    a = np.zeros(3, K.SECONDARY_SPHERE_DTYPE)
+    a[0] = (c + 0.10 * ext * np.array([1.0, 0.3, 0.2]), 0.08 * ext, K.MATERIAL_GLASS, 1.0, 1.3, (0.95, 0.9, 0.85))
+    a[1] = (c - 0.12 * ext * np.array([1.0, -0.2, 0.1]), 0.10 * ext, K.MATERIAL_MIRROR, 0.85, 1.0, (0.6, 0.7, 0.9))
+    a[2] = (c + 0.20 * ext * np.array([0.0, 1.0, 0.0]), 0.04 * ext, K.MATERIAL_DIFFUSE, 0.0, 1.0, (0.8, 0.2, 0.1))
+    return a
+
+
+def mesh_in_view(aabb):
+    """A mirror quad (two triangles) through the scene centre, tilted, and
+    a tetrahedron of four triangles beside it."""
+    lo, hi = np.asarray(aabb[:3], np.float64), np.asarray(aabb[3:], np.float64)
+    c, ext = 0.5 * (lo + hi), float((hi - lo).max())
+    s = 0.3 * ext
+    q = [c + s * np.array(v) for v in ([-1, -1, 0.3], [1, -1, -0.3], [1, 1, -0.3], [-1, 1, 0.3])]
+    t0 = c + 0.25 * ext * np.array([0.6, 0.4, 0.5])
+    tv = [t0 + 0.1 * ext * np.array(v) for v in ([1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1])]
+    tris = [q[0], q[1], q[2], q[0], q[2], q[3],
+            tv[0], tv[1], tv[2], tv[0], tv[3], tv[1], tv[0], tv[2], tv[3], tv[1], tv[3], tv[2]]
+    return np.asarray(tris, np.float32).reshape(-1, 9)
