@@ -17,7 +17,16 @@ struct State {
     kdgrt_scene* scene = nullptr;
     float aabb[6] = {0, 0, 0, 0, 0, 0}; // .kdt order
     int last_w = 1920, last_h = 1080;
+    // SECONDARY_RAY set-up: frames run the hybrid render
+    bool hybrid = false;
+    int sec_prim = KDGRT_SECONDARY_SPHERE;
+    float* d_mesh = nullptr;
+    int mesh_tris = 0;
+    int mesh_material = 1;
+    float mesh_reflectivity = 0.95f, mesh_ior = 1.1f, mesh_color[3] = {0.95f, 0.95f, 0.95f};
 };
+
+static_assert(sizeof(float3) == 12, "SecondaryMesh vertices are packed float triples");
 
 State& state() {
     static State s;
@@ -40,9 +49,15 @@ bool initCuda() {
     return true;
 }
 
-void renderGaussianWithCudaSetup(const CompositeObject& object, const std::vector<Gaussian>& gaussians) {
-    State& s = state();
-    std::lock_guard<std::mutex> lk(s.mu);
+static void drop_mesh(State& s) {
+    if (s.d_mesh) kdgrt_device_free(s.ctx, s.d_mesh);
+    s.d_mesh = nullptr;
+    s.mesh_tris = 0;
+    s.hybrid = false;
+}
+
+static void setup_locked(State& s, const CompositeObject& object, const std::vector<Gaussian>& gaussians) {
+    drop_mesh(s);
     const KdTree* t = object.kd_tree;
     if (!t || gaussians.empty()) { // cudaRenderer.cu:2571-2575
         std::fprintf(stderr, "[kdgrt] renderGaussianWithCudaSetup: Kd-tree or Gaussian data is empty.\n");
@@ -67,6 +82,62 @@ void renderGaussianWithCudaSetup(const CompositeObject& object, const std::vecto
         return;
     }
     for (int k = 0; k < 6; ++k) s.aabb[k] = aabb[k];
+}
+
+void renderGaussianWithCudaSetup(const CompositeObject& object, const std::vector<Gaussian>& gaussians) {
+    State& s = state();
+    std::lock_guard<std::mutex> lk(s.mu);
+    setup_locked(s, object, gaussians);
+}
+
+void renderGaussianWithCudaSetup(const CompositeObject& object, const std::vector<Gaussian>& gaussians,
+                                 SecondaryMesh& secMesh) {
+    State& s = state();
+    std::lock_guard<std::mutex> lk(s.mu);
+    setup_locked(s, object, gaussians);
+    if (!s.scene) return;
+    // initSecondaryMesh (cudaRenderer.cu:2064-2083): the vertices to device memory, the material by value
+    const size_t n_tri = secMesh.vert.size() / 3;
+    if (n_tri) {
+        void* d = nullptr;
+        if (kdgrt_device_alloc(s.ctx, n_tri * 9 * sizeof(float), &d) != KDGRT_OK ||
+            kdgrt_memcpy_h2d(s.ctx, d, secMesh.vert.data(), n_tri * 9 * sizeof(float)) != KDGRT_OK) {
+            report("renderGaussianWithCudaSetup (secondary mesh)");
+            if (d) kdgrt_device_free(s.ctx, d);
+            return;
+        }
+        s.d_mesh = static_cast<float*>(d);
+    }
+    s.mesh_tris = (int)n_tri;
+    s.mesh_material = secMesh.materialType;
+    s.mesh_reflectivity = secMesh.reflectivity;
+    s.mesh_ior = secMesh.ior;
+    s.mesh_color[0] = secMesh.color.x;
+    s.mesh_color[1] = secMesh.color.y;
+    s.mesh_color[2] = secMesh.color.z;
+    s.hybrid = true;
+}
+
+void setSecondaryPrim(int prim) {
+    State& s = state();
+    std::lock_guard<std::mutex> lk(s.mu);
+    s.sec_prim = prim == KDGRT_SECONDARY_TRI ? KDGRT_SECONDARY_TRI : KDGRT_SECONDARY_SPHERE;
+}
+
+bool loadSecondaryMesh(const std::string& filepath, SecondaryMesh& mesh) {
+    uint64_t n = 0;
+    if (kdgrt_obj_read_secondary(filepath.c_str(), nullptr, 0, &n) != KDGRT_OK) {
+        report("loadSecondaryMesh");
+        return false;
+    }
+    mesh.vert.assign(3 * n, float3{0.0f, 0.0f, 0.0f});
+    if (n && kdgrt_obj_read_secondary(filepath.c_str(), reinterpret_cast<float*>(mesh.vert.data()), n, &n) !=
+                 KDGRT_OK) {
+        report("loadSecondaryMesh");
+        return false;
+    }
+    mesh.triangleCount = (int)n;
+    return true;
 }
 
 static bool frame_info(const Camera& camera, int width, int height, kdgrt_camera_info* ci) {
@@ -97,9 +168,32 @@ float renderGaussianWithCudaFrame(const Camera& camera, int width, int height, f
         return -1.0f;
     }
     float ms = 0.0f;
-    if (kdgrt_timer_begin(s.ctx, stream) != KDGRT_OK ||
-        kdgrt_render(s.ctx, s.scene, &ci, width, height, 12, KDGRT_OUT_RGB_F32, d_framebuffer, stream) != KDGRT_OK ||
-        kdgrt_timer_end(s.ctx, stream, &ms) != KDGRT_OK) {
+    kdgrt_secondary sec{};
+    kdgrt_secondary_sphere demo[2];
+    if (s.hybrid) {
+        sec.prim = s.sec_prim;
+        sec.max_bounces = 5; // MAX_BOUNCES (cudaRenderer.cu:2230)
+        if (s.sec_prim == KDGRT_SECONDARY_SPHERE) {
+            kdgrt_secondary_demo_spheres(demo);
+            sec.count = 2;
+            sec.spheres = demo;
+        } else {
+            sec.count = s.mesh_tris;
+            sec.d_vertices = s.d_mesh;
+            sec.mesh_material = s.mesh_material;
+            sec.mesh_reflectivity = s.mesh_reflectivity;
+            sec.mesh_ior = s.mesh_ior;
+            for (int k = 0; k < 3; ++k) sec.mesh_color[k] = s.mesh_color[k];
+        }
+    }
+    if (kdgrt_timer_begin(s.ctx, stream) != KDGRT_OK) {
+        report("renderGaussianWithCudaFrame");
+        return -1.0f;
+    }
+    const kdgrt_status launched =
+        s.hybrid ? kdgrt_render_hybrid(s.ctx, s.scene, &ci, width, height, 12, &sec, d_framebuffer, stream)
+                 : kdgrt_render(s.ctx, s.scene, &ci, width, height, 12, KDGRT_OUT_RGB_F32, d_framebuffer, stream);
+    if (launched != KDGRT_OK || kdgrt_timer_end(s.ctx, stream, &ms) != KDGRT_OK) {
         report("renderGaussianWithCudaFrame");
         return -1.0f;
     }
@@ -134,6 +228,7 @@ void warmUp(float* d_framebuffer, hipStream_t stream) {
 void cleanupCudaResources() {
     State& s = state();
     std::lock_guard<std::mutex> lk(s.mu);
+    drop_mesh(s);
     if (s.scene) kdgrt_scene_destroy(s.scene);
     s.scene = nullptr;
     if (s.ctx) kdgrt_shutdown(s.ctx);

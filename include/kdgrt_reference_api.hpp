@@ -8,6 +8,9 @@
 //   renderGaussianWithCudaFrame(cam, w, h, d_fb, stream)  cudaRenderer.cu:3099-3484
 //   warmUp(d_fb, stream)                         cudaRenderer.cu:2532-2554
 //   cleanupCudaResources()                       cudaRenderer.cu:3486-3521
+// and, for the SECONDARY_RAY build,
+//   renderGaussianWithCudaSetup(obj, gaussians, secMesh)  cudaRenderer.cu:2557-2565, 2678-2680
+//   loadSecondaryMesh(path, mesh)                Kd-treeConverterMain.cpp:178-226
 // with the same structs (layout-compatible: Kd-treeConverter.h:282-335,
 // OpenGLStuffs.h:21-28), the same caller contract (Gaussians carry inverted
 // scales, Kd-treeConverterMain.cpp:3636-3642), the same framebuffer
@@ -21,6 +24,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
 #include <vector>
 
 #include "kdgrt.h"
@@ -74,11 +78,38 @@ struct Camera {
     double fovy, aspect, near_c, far_c;
 };
 
+// cudaRenderer.h:11-19 (SECONDARY_RAY builds): the secondary mesh and its
+// one material (1 reflect, 2 refract, other: opaque colour).
+struct SecondaryMesh {
+    std::vector<float3> vert;
+    int triangleCount = 0;
+    int materialType = 1;
+    float reflectivity = 0.95f;
+    float ior = 1.1f;
+    float3 color = {0.95f, 0.95f, 0.95f};
+};
+
 KDGRT_REF_API bool initCuda();
 KDGRT_REF_API void renderGaussianWithCudaSetup(const CompositeObject& object, const std::vector<Gaussian>& gaussians);
 KDGRT_REF_API float renderGaussianWithCudaFrame(const Camera& camera, int width, int height, float* d_framebuffer,
                                   hipStream_t stream);
 KDGRT_REF_API void warmUp(float* d_framebuffer, hipStream_t stream);
+
+// SECONDARY_RAY setup: the scene plus the secondary mesh (initSecondaryMesh,
+// cudaRenderer.cu:2064-2083); frames then run the hybrid render
+// (renderKernelGaussian_Mesh_Hybrid, cudaRenderer.cu:2217-2311, MAX_BOUNCES 5)
+// into the same framebuffer.
+KDGRT_REF_API void renderGaussianWithCudaSetup(const CompositeObject& object, const std::vector<Gaussian>& gaussians,
+                                               SecondaryMesh& secMesh);
+// SECONDARY_PRIM (Kd-treeConverter.h:28-30) is a compile-time switch in the
+// reference; here it is chosen at run time: KDGRT_SECONDARY_SPHERE (the
+// reference's setting: its two demo spheres, cudaRenderer.cu:2162-2167) or
+// KDGRT_SECONDARY_TRI (the mesh given at set-up).
+KDGRT_REF_API void setSecondaryPrim(int prim);
+// OBJ triangles as the secondary mesh, vertices stored as (x, -z, y);
+// returns false when the file cannot be read or holds faces of more than four
+// vertices.
+KDGRT_REF_API bool loadSecondaryMesh(const std::string& filepath, SecondaryMesh& mesh);
 KDGRT_REF_API void cleanupCudaResources();
 
 // Handles of the singleton state, for callers that mix both APIs.

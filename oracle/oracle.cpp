@@ -588,6 +588,25 @@ void render_hybrid(const Scene& sc, const SecScene& s, const CamInfo& cam, int W
         }
 }
 
+// Hybrid paths from explicit primary rays (f32 origins and directions, used
+// as given): out_rgb n*3, min(c, 1).
+template <class T>
+void render_hybrid_rays(const Scene& sc, const SecScene& s, const float* origins, const float* dirs, uint64_t n,
+                        int D, T* out_rgb, int32_t* segments, int threads) {
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long i = 0; i < (long long)n; ++i) {
+        const V3<T> o = {T(origins[3 * i]), T(origins[3 * i + 1]), T(origins[3 * i + 2])};
+        const V3<T> d = {T(dirs[3 * i]), T(dirs[3 * i + 1]), T(dirs[3 * i + 2])};
+        int nseg = 0;
+        const V3<T> c = hybrid_pixel<T>(sc, s, o, d, D, nseg);
+        out_rgb[3 * i + 0] = tmin(c.x, T(1));
+        out_rgb[3 * i + 1] = tmin(c.y, T(1));
+        out_rgb[3 * i + 2] = tmin(c.z, T(1));
+        if (segments) segments[i] = nseg;
+    }
+}
+
 Scene make_scene(const void* nodes, const uint32_t* offsets, const void* g, const float* aabb) {
     Scene sc;
     sc.nodes = (const Node*)nodes;
@@ -666,6 +685,22 @@ void oracle_render_hybrid_f64(const void* nodes, const uint32_t* offsets, const 
     render_hybrid<double>(make_scene(nodes, offsets, g, aabb),
                           make_sec(prim, count, spheres, tri, mesh, max_bounces), *(const CamInfo*)cam, W, H, D,
                           out_rgb, segments, threads);
+}
+void oracle_render_hybrid_rays_f32(const void* nodes, const uint32_t* offsets, const void* g, const float* aabb,
+                                   const float* origins, const float* dirs, uint64_t n, int D, int prim, int count,
+                                   const void* spheres, const float* tri, const float* mesh, int max_bounces,
+                                   float* out_rgb, int32_t* segments, int threads) {
+    render_hybrid_rays<float>(make_scene(nodes, offsets, g, aabb),
+                              make_sec(prim, count, spheres, tri, mesh, max_bounces), origins, dirs, n, D, out_rgb,
+                              segments, threads);
+}
+void oracle_render_hybrid_rays_f64(const void* nodes, const uint32_t* offsets, const void* g, const float* aabb,
+                                   const float* origins, const float* dirs, uint64_t n, int D, int prim, int count,
+                                   const void* spheres, const float* tri, const float* mesh, int max_bounces,
+                                   double* out_rgb, int32_t* segments, int threads) {
+    render_hybrid_rays<double>(make_scene(nodes, offsets, g, aabb),
+                               make_sec(prim, count, spheres, tri, mesh, max_bounces), origins, dirs, n, D, out_rgb,
+                               segments, threads);
 }
 // Segments of one hybrid path through the secondary primitives alone (no
 // Gaussians): up to max_bounces records of origin, direction and max_t

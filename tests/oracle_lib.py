@@ -155,3 +155,22 @@ def hybrid_path(origin, direction, spheres=None, triangles=None, mesh=(1, 0.95, 
         n = lib().oracle_hybrid_path_f32(C.c_int(0), C.c_int(len(tri)), None, _p(tri), _p(m), C.c_int(max_bounces),
                                          _p(o), _p(d), _p(path), _p(rgb))
     return path[:n], rgb
+
+
+def render_hybrid_rays(nodes, offsets, g_inv, aabb, origins, dirs, spheres, max_bounces=5, depth=12, f64=False,
+                       threads=0):
+    """Hybrid paths from explicit primary rays (secondary spheres only):
+    rgb (n, 3) clamped to 1, segments (n,)."""
+    nodes = np.ascontiguousarray(nodes)
+    offsets = np.ascontiguousarray(offsets, np.uint32)
+    aabb = np.ascontiguousarray(aabb, np.float32)
+    origins = np.ascontiguousarray(origins, np.float32)
+    dirs = np.ascontiguousarray(dirs, np.float32)
+    sph = np.ascontiguousarray(spheres)
+    n = len(origins)
+    out = np.zeros((n, 3), np.float64 if f64 else np.float32)
+    seg = np.zeros(n, np.int32)
+    fn = lib().oracle_render_hybrid_rays_f64 if f64 else lib().oracle_render_hybrid_rays_f32
+    fn(_p(nodes), _p(offsets), _p(g_inv), _p(aabb), _p(origins), _p(dirs), C.c_uint64(n), C.c_int(depth), C.c_int(1),
+       C.c_int(len(sph)), _p(sph), None, None, C.c_int(max_bounces), _p(out), _p(seg), C.c_int(threads))
+    return {"rgb": out, "segments": seg}

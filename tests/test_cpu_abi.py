@@ -29,6 +29,21 @@ def test_every_declared_symbol_is_exported():
     assert not missing, missing
 
 
+def test_reference_named_cpp_api_is_exported():
+    """Every function of include/kdgrt_reference_api.hpp is a default-
+    visibility symbol of the library (demangled names from nm -DC)."""
+    import subprocess
+    out = subprocess.run(["nm", "-DC", K.LIB_PATH], stdout=subprocess.PIPE, text=True, check=True).stdout
+    txt = open(os.path.join(ROOT, "include", "kdgrt_reference_api.hpp")).read()
+    declared = set(re.findall(r"KDGRT_REF_API\s+[\w:<>\s\*&]+?\b(\w+)\s*\(", txt))
+    assert {"initCuda", "renderGaussianWithCudaSetup", "renderGaussianWithCudaFrame", "warmUp",
+            "cleanupCudaResources", "setSecondaryPrim", "loadSecondaryMesh"} <= declared
+    exported = set(re.findall(r" T kdgrt_ref::(\w+)\(", out))
+    assert declared <= exported, declared - exported
+    # both set-up overloads (plain and SECONDARY_RAY)
+    assert out.count("kdgrt_ref::renderGaussianWithCudaSetup(") == 2
+
+
 def test_library_is_built_for_gfx950():
     data = open(K.LIB_PATH, "rb").read()
     assert b"gfx950" in data

@@ -20,6 +20,6 @@ def test_reference_named_host_api(tmp_path):
                     os.path.join(ROOT, "tests", "cpp", "host_api_driver.cpp"), "-o", exe,
                     "-L" + os.path.dirname(K.LIB_PATH), "-lkdgrt", "-L/opt/rocm/lib", "-lamdhip64",
                     "-Wl,-rpath," + os.path.dirname(K.LIB_PATH) + ":/opt/rocm/lib"], check=True)
-    r = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    r = subprocess.run([exe, str(tmp_path)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0 and "host_api ok=1" in r.stdout
