@@ -290,6 +290,13 @@ def obj_read_secondary(path):
     return out
 
 
+def write_png(path, pixels):
+    """PNG file of 8-bit pixels, shape (h, w, 3 or 4), rows top to bottom."""
+    px = np.ascontiguousarray(pixels, np.uint8)
+    h, w, c = px.shape
+    check(lib().kdgrt_png_write(str(path).encode(), C.c_int(w), C.c_int(h), C.c_int(c), _p(px)))
+
+
 def write_stats(counters, width, height, directory):
     """Writes the reference's per-ray statistics files from render_debug
     counters (shape (height, width, 8) or (height*width, 8), ray order)."""
@@ -424,6 +431,18 @@ class Device:
                 if isinstance(k, DeviceBuffer):
                     self.sync(stream)
                     k.free()
+
+    def framebuffer_to_u8(self, fb, w, h, in_channels, out_channels=3, flip_rows=True, stream=None):
+        """8-bit pixels (h, w, out_channels) of a device framebuffer, converted
+        on the device as the reference's PNG dump does."""
+        out = self.alloc(w * h * out_channels)
+        try:
+            check(lib().kdgrt_framebuffer_to_u8(self.h, fb.ptr, C.c_int(w), C.c_int(h), C.c_int(in_channels), out.ptr,
+                                                C.c_int(out_channels), C.c_int(1 if flip_rows else 0), stream))
+            self.sync(stream)
+            return out.download(np.uint8, (h, w, out_channels))
+        finally:
+            out.free()
 
     def render_debug(self, scene, ci, w, h, depth=12, seq_cap=0):
         out = self.alloc(w * h * 16)

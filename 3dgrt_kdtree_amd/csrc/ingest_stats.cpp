@@ -5,7 +5,9 @@
 //    (cudaRenderer.cu:3173-3400): renderStats.txt, render_stats.csv,
 //    heatmap_*.csv, isectCnt_kd.bin, blendCnt_kd.bin;
 //  * the secondary mesh of the hybrid render from a Wavefront OBJ file
-//    (loadSecondaryMesh, Kd-treeConverterMain.cpp:178-226).
+//    (loadSecondaryMesh, Kd-treeConverterMain.cpp:178-226);
+//  * PNG files of rendered frames (the reference's saveDeviceFramebufferToPNG,
+//    cudaRenderer.cu:3066-3096, writes them with stb_image_write).
 #include <algorithm>
 #include <cctype>
 #include <cerrno>
@@ -196,9 +198,86 @@ bool obj_vertex_index(const char* tok, size_t n_vertices, size_t& out) {
     return true;
 }
 
+// CRC-32 (ISO-HDLC, reflected polynomial 0xEDB88320) of PNG chunks.
+uint32_t crc32(const uint8_t* p, size_t n, uint32_t c = 0) {
+    static uint32_t table[256];
+    static bool ready = false;
+    if (!ready) {
+        for (uint32_t i = 0; i < 256; ++i) {
+            uint32_t v = i;
+            for (int k = 0; k < 8; ++k) v = (v & 1) ? 0xEDB88320u ^ (v >> 1) : v >> 1;
+            table[i] = v;
+        }
+        ready = true;
+    }
+    c = ~c;
+    for (size_t i = 0; i < n; ++i) c = table[(c ^ p[i]) & 0xFF] ^ (c >> 8);
+    return ~c;
+}
+
+void put_be32(std::vector<uint8_t>& b, uint32_t v) {
+    for (int k = 3; k >= 0; --k) b.push_back((uint8_t)(v >> (8 * k)));
+}
+
+void png_chunk(std::vector<uint8_t>& out, const char* type, const std::vector<uint8_t>& data) {
+    put_be32(out, (uint32_t)data.size());
+    std::vector<uint8_t> td(type, type + 4);
+    td.insert(td.end(), data.begin(), data.end());
+    out.insert(out.end(), td.begin(), td.end());
+    put_be32(out, crc32(td.data(), td.size()));
+}
+
 } // namespace
 
 extern "C" {
+
+kdgrt_status kdgrt_png_write(const char* path, int width, int height, int channels, const uint8_t* pixels) {
+    if (!path || !pixels || width <= 0 || height <= 0 || (channels != 3 && channels != 4))
+        return set_error(KDGRT_ERR_INVALID_ARGUMENT, "png_write: bad argument");
+    const size_t row = (size_t)width * channels;
+    // scanlines with filter type 0, as one zlib stream of stored blocks
+    std::vector<uint8_t> raw;
+    raw.reserve((row + 1) * height);
+    for (int y = 0; y < height; ++y) {
+        raw.push_back(0);
+        raw.insert(raw.end(), pixels + (size_t)y * row, pixels + (size_t)(y + 1) * row);
+    }
+    std::vector<uint8_t> z = {0x78, 0x01};
+    for (size_t off = 0; off < raw.size() || off == 0; off += 65535) {
+        const size_t len = std::min<size_t>(65535, raw.size() - off);
+        const bool last = off + len >= raw.size();
+        z.push_back(last ? 1 : 0);
+        z.push_back((uint8_t)(len & 0xFF));
+        z.push_back((uint8_t)(len >> 8));
+        z.push_back((uint8_t)(~len & 0xFF));
+        z.push_back((uint8_t)((~len >> 8) & 0xFF));
+        z.insert(z.end(), raw.begin() + off, raw.begin() + off + len);
+        if (last) break;
+    }
+    uint32_t a = 1, b = 0; // Adler-32
+    for (uint8_t v : raw) {
+        a = (a + v) % 65521u;
+        b = (b + a) % 65521u;
+    }
+    put_be32(z, (b << 16) | a);
+    std::vector<uint8_t> out = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1A, '\n'};
+    std::vector<uint8_t> ihdr;
+    put_be32(ihdr, (uint32_t)width);
+    put_be32(ihdr, (uint32_t)height);
+    ihdr.push_back(8);                       // bit depth
+    ihdr.push_back(channels == 4 ? 6 : 2);   // colour type RGBA / RGB
+    ihdr.push_back(0);
+    ihdr.push_back(0);
+    ihdr.push_back(0);
+    png_chunk(out, "IHDR", ihdr);
+    png_chunk(out, "IDAT", z);
+    png_chunk(out, "IEND", {});
+    FILE* fp = std::fopen(path, "wb");
+    if (!fp) return set_error(KDGRT_ERR_IO, std::string("png_write: cannot open ") + path);
+    const bool ok = std::fwrite(out.data(), 1, out.size(), fp) == out.size();
+    std::fclose(fp);
+    return ok ? KDGRT_OK : set_error(KDGRT_ERR_IO, std::string("png_write: short write to ") + path);
+}
 
 kdgrt_status kdgrt_obj_read_secondary(const char* path, float* out_xyz, uint64_t capacity, uint64_t* n_triangles) {
     if (!path || !n_triangles) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "obj_read_secondary: null argument");

@@ -10,6 +10,8 @@
 //              from many views / ray sets / scenes; one queue per XCD with
 //              stealing, each wave pulls its own tile (no block barriers).
 //  k_reset     zeroes the queue counters (graph-safe, no memset node).
+//  k_to_u8     framebuffer to 8-bit pixels (saveDeviceFramebufferToPNG,
+//              cudaRenderer.cu:3066-3096) on the device.
 //  k_hybrid    hybrid view (SECONDARY_RAY, cudaRenderer.cu:2217-2311): per
 //              pixel, segments of the Gaussian trace between mirror / glass
 //              secondary primitives, tiled as k_view.
@@ -262,6 +264,25 @@ struct BatchArgs {
 };
 
 // Plain streaming copy, the calibration kernel for HBM traffic counters.
+// One output pixel per thread: clamp to [0, 1], times 255, truncated
+// (cudaRenderer.cu:3083-3086); flip: read framebuffer row h-1-y for image
+// row y (the reference's stbi_flip_vertically_on_write); a fourth output
+// channel is 255 (opaque), a fourth input channel (T) is dropped.
+__global__ __launch_bounds__(256) void k_to_u8(const float* __restrict__ fb, int w, int h, int cin,
+                                               uint8_t* __restrict__ out, int cout, int flip) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (uint64_t)w * h) return;
+    const int x = (int)(i % (uint64_t)w), y = (int)(i / (uint64_t)w);
+    const int sy = flip ? h - 1 - y : y;
+    const float* src = fb + (size_t)cin * ((size_t)sy * w + x);
+    uint8_t* dst = out + (size_t)cout * i;
+    for (int c = 0; c < 3; ++c) {
+        const float v = fmaxf(0.0f, fminf(1.0f, src[c]));
+        dst[c] = (uint8_t)(v * 255.0f);
+    }
+    if (cout == 4) dst[3] = 255;
+}
+
 __global__ __launch_bounds__(256) void k_copy(float4* __restrict__ dst, const float4* __restrict__ src, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) dst[i] = src[i];
 }
@@ -730,6 +751,20 @@ kdgrt_status kdgrt_copy_f32x4(kdgrt_context* ctx, void* dst, const void* src, ui
     const unsigned grid = (unsigned)std::min<uint64_t>(want, (uint64_t)n_cu * 16);
     hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, (hipStream_t)stream, (float4*)dst, (const float4*)src,
                        n_float4);
+    HIP_TRY(hipGetLastError());
+    return KDGRT_OK;
+}
+
+kdgrt_status kdgrt_framebuffer_to_u8(kdgrt_context* ctx, const float* fb, int width, int height, int in_channels,
+                                     uint8_t* out, int out_channels, int flip_rows, void* stream) {
+    if (!ctx || !fb || !out) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "framebuffer_to_u8: null");
+    if (width <= 0 || height <= 0 || width > 32768 || height > 32768)
+        return set_error(KDGRT_ERR_INVALID_ARGUMENT, "framebuffer_to_u8: bad size");
+    if ((in_channels != 3 && in_channels != 4) || (out_channels != 3 && out_channels != 4))
+        return set_error(KDGRT_ERR_INVALID_ARGUMENT, "framebuffer_to_u8: channels must be 3 or 4");
+    const uint64_t n = (uint64_t)width * height;
+    hipLaunchKernelGGL(k_to_u8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, fb, width,
+                       height, in_channels, out, out_channels, flip_rows ? 1 : 0);
     HIP_TRY(hipGetLastError());
     return KDGRT_OK;
 }

@@ -31,6 +31,8 @@
  *   kdgrt_render_hybrid    <- renderGaussianWithCudaFrame() with SECONDARY_RAY
  *                             (kernel renderKernelGaussian_Mesh_Hybrid, :2217-2311)
  *   kdgrt_obj_read_secondary <- loadSecondaryMesh()            Kd-treeConverterMain.cpp:178-226
+ *   kdgrt_framebuffer_to_u8 + kdgrt_png_write
+ *                          <- saveDeviceFramebufferToPNG()     cudaRenderer.cu:3066-3096
  *   kdgrt_render_rays, kdgrt_batch_*, kdgrt_server_* have no single
  *   reference counterpart: they run the same per-ray path over explicit ray
  *   lists, many views in one persistent launch, and a serving loop.
@@ -287,6 +289,17 @@ KDGRT_API kdgrt_status kdgrt_render_hybrid(kdgrt_context* ctx, const kdgrt_scene
                                            const kdgrt_camera_info* cam, int width, int height, int stack_depth,
                                            const kdgrt_secondary* secondary, float* out_rgb, void* stream);
 
+/* ---------------- frame output ---------------- */
+/* Device conversion of a framebuffer (width*height pixels of in_channels
+ * floats: 3 = KDGRT_OUT_RGB_F32, 4 = KDGRT_OUT_RGBT_F32) to 8-bit pixels of
+ * out_channels (3 = RGB, 4 = RGBA with alpha 255): clamp to [0, 1], times
+ * 255, truncated (cudaRenderer.cu:3083-3086).  flip_rows != 0 writes image
+ * row y from framebuffer row height-1-y, undoing the framebuffer's row flip
+ * as the reference's PNG dump does; out is device memory. */
+KDGRT_API kdgrt_status kdgrt_framebuffer_to_u8(kdgrt_context* ctx, const float* fb, int width, int height,
+                                               int in_channels, uint8_t* out, int out_channels, int flip_rows,
+                                               void* stream);
+
 /* ---------------- batched persistent launch ---------------- */
 typedef struct kdgrt_view {
     const kdgrt_scene* scene;
@@ -446,6 +459,10 @@ KDGRT_API kdgrt_status kdgrt_cameras_from_nerf_json(const char* path, double asp
  * and sets *n_triangles; pass out_xyz = NULL to count. */
 KDGRT_API kdgrt_status kdgrt_obj_read_secondary(const char* path, float* out_xyz, uint64_t capacity,
                                                 uint64_t* n_triangles);
+/* PNG file of 8-bit pixels in host memory (rows top to bottom, channels
+ * 3 = RGB or 4 = RGBA), 8 bits per channel, zlib stored blocks. */
+KDGRT_API kdgrt_status kdgrt_png_write(const char* path, int width, int height, int channels,
+                                       const uint8_t* pixels);
 /* Per-ray statistics files of a debug render (the HIT_AND_NODE_COUNT_DEBUG
  * products, cudaRenderer.cu:3173-3400) into directory `dir`: renderStats.txt,
  * render_stats.csv, heatmap_{node_visits,leaf_visits,intersection,hits_found,

@@ -70,3 +70,40 @@ def test_stats_files_by_hand(tmp_path):
     assert (tmp_path / "heatmap_max_sort_size.csv").read_text() == "2,4,1\n1,0,4\n"
     for name in ("isectCnt_kd.bin", "blendCnt_kd.bin"):
         assert np.array_equal(np.fromfile(tmp_path / name, np.uint32), c[..., 3].reshape(-1).astype(np.uint32))
+
+
+def _png_decode(data):
+    """Minimal PNG reader for the writer's output: chunks with CRCs, one
+    zlib stream, filter type 0 scanlines."""
+    import struct
+    import zlib
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, chunks = 8, {}
+    while pos < len(data):
+        (n,) = struct.unpack(">I", data[pos:pos + 4])
+        typ, body = data[pos + 4:pos + 8], data[pos + 8:pos + 8 + n]
+        (crc,) = struct.unpack(">I", data[pos + 8 + n:pos + 12 + n])
+        assert zlib.crc32(typ + body) == crc, typ
+        chunks.setdefault(typ, b"")
+        chunks[typ] += body
+        pos += 12 + n
+    w, h, depth, ctype = struct.unpack(">IIBB", chunks[b"IHDR"][:10])
+    ch = {2: 3, 6: 4}[ctype]
+    raw = zlib.decompress(chunks[b"IDAT"])
+    rows = np.frombuffer(raw, np.uint8).reshape(h, 1 + w * ch)
+    assert (rows[:, 0] == 0).all() and depth == 8 and b"IEND" in chunks
+    return rows[:, 1:].reshape(h, w, ch)
+
+
+@pytest.mark.parametrize("shape", [(3, 5, 3), (37, 700, 4), (1, 1, 3), (130, 260, 3)])
+def test_png_round_trip(tmp_path, shape):
+    rng = np.random.default_rng(sum(shape))
+    px = rng.integers(0, 256, shape, dtype=np.uint8)
+    p = tmp_path / "f.png"
+    K.write_png(p, px)
+    assert np.array_equal(_png_decode(p.read_bytes()), px)
+
+
+def test_png_rejects_bad_channels(tmp_path):
+    with pytest.raises(K.KdgrtError):
+        K.write_png(tmp_path / "x.png", np.zeros((2, 2, 2), np.uint8))

@@ -197,3 +197,30 @@ def test_full_size_1m_1080p(dev):
     img = dev.render_image(sc, ci, 1920, 1080)
     assert np.array_equal(img, gpu["rgbt"])
     sc.free()
+
+
+@pytest.mark.parametrize("fmt,cin,cout,flip", [(K.OUT_RGB, 3, 3, True), (K.OUT_RGBT, 4, 4, True),
+                                                (K.OUT_RGB, 3, 4, False)])
+def test_framebuffer_to_u8(dev, tmp_path, fmt, cin, cout, flip):
+    """Frame output (saveDeviceFramebufferToPNG, cudaRenderer.cu:3066-3096):
+    clamp to [0, 1], times 255 in f32, truncated, rows flipped back; then the
+    PNG file of it."""
+    hs = S.uniform_10k()
+    ci = S.cam_c0(200, 120)
+    sc = dev.scene(hs.tree, hs.g)
+    fb = dev.alloc(200 * 120 * cin * 4)
+    dev.render(sc, ci, 200, 120, fb, fmt=fmt)
+    dev.sync()
+    host = fb.download(np.float32, (120, 200, cin))
+    px = dev.framebuffer_to_u8(fb, 200, 120, cin, cout, flip_rows=flip)
+    ref = (np.clip(host[..., :3], 0, 1) * np.float32(255)).astype(np.uint8)
+    if flip:
+        ref = ref[::-1]
+    assert np.array_equal(px[..., :3], ref)
+    if cout == 4:
+        assert (px[..., 3] == 255).all()
+    assert 0 < px[..., :3].mean() < 255
+    K.write_png(tmp_path / "frame.png", px)
+    assert (tmp_path / "frame.png").stat().st_size > 200 * 120 * cout
+    fb.free()
+    sc.free()
