@@ -11,6 +11,7 @@ Reference entry points mirrored (Kd-treeConverter_v0.9/Kd-treeConverter/):
   Renderer.init()             <- initCuda()                     cudaRenderer.cu:2438
   Renderer.setup(...)         <- renderGaussianWithCudaSetup()  cudaRenderer.cu:2557
   Renderer.frame(...)         <- renderGaussianWithCudaFrame()  cudaRenderer.cu:3099
+  Renderer.warm_up(...)       <- warmUp()                       cudaRenderer.cu:2532
   Renderer.cleanup()          <- cleanupCudaResources()         cudaRenderer.cu:3486
   build_tree(...)             <- build_kd_tree_for_composite_object  Kd-treeConverter.cpp:22
   Tree.write_kdt / read_kdt   <- dump_/read_kd_tree_...          Kd-treeConverter.cpp:297, 444
@@ -535,6 +536,70 @@ class Scene:
         if self.h:
             lib().kdgrt_scene_destroy(self.h)
             self.h = None
+
+
+class Renderer:
+    """The reference renderer's host API (cudaRenderer.h:60-109) for Python
+    callers, on the C ABI: init() <- initCuda, setup() <-
+    renderGaussianWithCudaSetup (Gaussians carry inverted scales, the
+    caller contract of Kd-treeConverterMain.cpp:3636-3642; with `secondary`
+    it is the SECONDARY_RAY set-up), frame() <- renderGaussianWithCudaFrame
+    (w*h*3 float RGB into a caller-owned device buffer, rows flipped, blocks,
+    returns kernel-only FPS), warm_up() <- warmUp, cleanup() <-
+    cleanupCudaResources."""
+
+    def __init__(self):
+        self.dev = None
+        self.scene = None
+        self.secondary = None
+
+    def init(self, device=0):
+        if self.dev is None:
+            self.dev = Device(device)
+        return True
+
+    def setup(self, tree, gaussians, secondary=None):
+        """secondary: None (plain frames), or a dict for hybrid frames:
+        {"spheres": array of SECONDARY_SPHERE_DTYPE} or {"triangles": (n, 9)
+        float32, "mesh_material", "mesh_reflectivity", "mesh_ior",
+        "mesh_color"}; "max_bounces" defaults to the reference's 5."""
+        if self.dev is None:
+            raise KdgrtError(1, "setup: init() was not called")
+        if len(gaussians) == 0:
+            raise KdgrtError(1, "setup: Kd-tree or Gaussian data is empty")
+        if self.scene is not None:
+            self.scene.free()
+        self.scene = Scene(self.dev, tree, gaussians, scales_inverted=True)
+        self.secondary = dict(secondary) if secondary is not None else None
+
+    def frame(self, camera, width, height, fb, stream=None):
+        if self.scene is None:
+            raise KdgrtError(1, "frame: no scene set up")
+        ci = camera_make(camera, width, height)
+        self.dev.timer_begin(stream)
+        if self.secondary is None:
+            self.dev.render(self.scene, ci, width, height, fb, fmt=OUT_RGB, stream=stream)
+        else:
+            self.dev.render_hybrid(self.scene, ci, width, height, out=fb, stream=stream, **self.secondary)
+        ms = self.dev.timer_end(stream)
+        self.dev.sync(stream)
+        return 1000.0 / ms if ms > 0 else 0.0
+
+    def warm_up(self, stream=None):
+        """One plain frame from a corner of the scene box into a scratch
+        buffer (warmUp renders the plain kernel once to load the code object
+        and fault in the scene)."""
+        if self.scene is None:
+            raise KdgrtError(1, "warm_up: no scene set up")
+        check(lib().kdgrt_warm_up(self.dev.h, self.scene.h, stream))
+
+    def cleanup(self):
+        if self.scene is not None:
+            self.scene.free()
+            self.scene = None
+        if self.dev is not None:
+            self.dev.close()
+            self.dev = None
 
 
 def default_tuning(dev=None):

@@ -77,3 +77,16 @@ def test_all_culled_scene_is_an_error():
     g["opacity"][:] = 0.001
     with pytest.raises(K.KdgrtError):
         K.Tree.build(g)
+
+
+def test_python_renderer_call_order_errors():
+    """K.Renderer reports the reference's call-order errors as exceptions
+    (setup before init, frame before setup) without touching a device."""
+    r = K.Renderer()
+    with pytest.raises(K.KdgrtError):
+        r.setup(None, [])
+    with pytest.raises(K.KdgrtError):
+        r.frame(K.Camera(), 8, 8, None)
+    with pytest.raises(K.KdgrtError):
+        r.warm_up()
+    r.cleanup()
