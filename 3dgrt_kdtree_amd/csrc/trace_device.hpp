@@ -175,17 +175,21 @@ __device__ __forceinline__ void sh_radiance(const float4* __restrict__ sh, const
 // t lies within r of the projection of the centre; kappa*|a|_1 covers the
 // float rounding of both the exact test and this one.  r = inf disables the
 // pre-test for a Gaussian (singular or non-orthogonal rotation).  The exact
-// test alone decides acceptance, so the pre-test changes no result.
+// test alone decides acceptance, so the pre-test changes no result.  The
+// pre-tests use fused multiply-adds: their margins bound the rounding of
+// separate products and sums, and a fused operation rounds once where those
+// round twice, so its error stays within the same bounds.
 __device__ __forceinline__ bool sphere_pass(const float o[3], const float d[3], const float4 sp, float t_near,
                                             float t_far) {
     const float ax = sp.x - o[0], ay = sp.y - o[1], az = sp.z - o[2];
     const uint32_t w = __float_as_uint(sp.w);
     const float r = __half2float(__ushort_as_half((unsigned short)(w & 0xFFFFu)));
     const float kap = __half2float(__ushort_as_half((unsigned short)(w >> 16)));
-    const float R = r + kap * (fabsf(ax) + fabsf(ay) + fabsf(az));
-    const float tp = ax * d[0] + ay * d[1] + az * d[2];
-    const float cx = ay * d[2] - az * d[1], cy = az * d[0] - ax * d[2], cz = ax * d[1] - ay * d[0];
-    const float dist2 = cx * cx + cy * cy + cz * cz;
+    const float R = __builtin_fmaf(kap, fabsf(ax) + fabsf(ay) + fabsf(az), r);
+    const float tp = __builtin_fmaf(az, d[2], __builtin_fmaf(ay, d[1], ax * d[0]));
+    const float cx = __builtin_fmaf(ay, d[2], -(az * d[1])), cy = __builtin_fmaf(az, d[0], -(ax * d[2])),
+                cz = __builtin_fmaf(ax, d[1], -(ay * d[0]));
+    const float dist2 = __builtin_fmaf(cz, cz, __builtin_fmaf(cy, cy, cx * cx));
     return (dist2 <= R * R) & (tp + R >= t_near) & (tp - R <= t_far);
 }
 
@@ -193,9 +197,10 @@ __device__ __forceinline__ bool sphere_pass(const float o[3], const float d[3], 
 // a = centre - o (the f32 difference the sphere test forms) and a threshold
 // q: every candidate that sphere_pass accepts has a.d >= q, so testing
 // a.d >= q instead keeps all of them (q is derived and rounded down on the
-// device by k_cone_prep).  a.d is formed in sphere_pass's order.
+// device by k_cone_prep; its margin 4 eps |a| covers the rounding of this
+// dot product).
 __device__ __forceinline__ bool cone_pass(const float d[3], const float4 c) {
-    const float tp = c.x * d[0] + c.y * d[1] + c.z * d[2];
+    const float tp = __builtin_fmaf(c.z, d[2], __builtin_fmaf(c.y, d[1], c.x * d[0]));
     return tp >= c.w;
 }
 
