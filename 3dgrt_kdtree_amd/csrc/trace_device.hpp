@@ -269,13 +269,18 @@ __device__ __forceinline__ TestOut ray_gaussian(const float o[3], const float d[
 // slots arrive in increasing order, so "after every entry with t <= r.t" is
 // the stable insertion-sort position.  An entry holds t, grayDist, opacity
 // and the Gaussian ID.  A hit that does not fit (or evicts the last entry)
-// marks the buffer as overflowed.
+// marks the buffer as overflowed.  Entries no lane of the wave holds are
+// skipped with wave-uniform branches: most leaf visits hold zero or one hit,
+// so most insertions touch only the first entries.
 template <int K>
 __device__ __forceinline__ void insert_hit(float (&kt)[K], float (&kg)[K], float (&ko)[K], uint32_t (&kid)[K],
                                            int& n, bool& overflow, const TestOut& r, uint32_t gid) {
     int pos = 0;
 #pragma unroll
-    for (int i = 0; i < K; ++i) pos += (i < n && kt[i] <= r.t) ? 1 : 0;
+    for (int i = 0; i < K; ++i) {
+        if (!__any(i < n)) break; // no lane of the wave holds entry i
+        pos += (i < n && kt[i] <= r.t) ? 1 : 0;
+    }
     if (pos >= K) {
         overflow = true;
         return;
@@ -283,6 +288,7 @@ __device__ __forceinline__ void insert_hit(float (&kt)[K], float (&kg)[K], float
     if (n == K) overflow = true;
 #pragma unroll
     for (int i = K - 1; i > 0; --i) {
+        if (!__any(i <= n)) continue; // entry i stays unused in every lane
         if (i > pos) {
             kt[i] = kt[i - 1];
             kg[i] = kg[i - 1];
