@@ -456,14 +456,14 @@ __device__ __forceinline__ void leaf(RayState& st, const DevScene& sc, RayDbg* d
         }
         int n = 0;
         bool overflow = false;
-        // Windows of 64 slots: the cheap pre-test runs for every slot and
+        // Windows of 32 slots: the cheap pre-test runs for every slot and
         // builds a survivor mask; the exact test then runs only for
         // survivors, in slot order, so a wavefront executes the exact test as
         // often as its busiest lane needs it rather than once for every slot
         // some lane still needs.
-        for (uint32_t w0 = 0; w0 < count; w0 += 64) {
-            const uint32_t wn = (count - w0) < 64u ? (count - w0) : 64u;
-            uint64_t mask = 0;
+        for (uint32_t w0 = 0; w0 < count; w0 += 32) {
+            const uint32_t wn = (count - w0) < 32u ? (count - w0) : 32u;
+            uint32_t mask = 0;
             // four slots per step, one contiguous read; the group may run past
             // the leaf (the record arrays are padded) and bits at or above wn
             // are cleared afterwards
@@ -482,18 +482,18 @@ __device__ __forceinline__ void leaf(RayState& st, const DevScene& sc, RayDbg* d
                            (sphere_pass(o, d, s2, st.t_near, st.t_far) ? 4u : 0u) |
                            (sphere_pass(o, d, s3, st.t_near, st.t_far) ? 8u : 0u);
                 }
-                mask |= (uint64_t)bits << j0;
+                mask |= bits << j0;
             }
-            if (wn < 64) mask &= (1ull << wn) - 1ull;
+            if (wn < 32) mask &= (1u << wn) - 1u;
             if (PROF) dbg->pretests += wn;
             KD_PROF_MARK(1);
-            if (PROF) dbg->survivors += (uint32_t)__popcll(mask);
+            if (PROF) dbg->survivors += (uint32_t)__popc(mask);
             // exact tests, two survivors per step so that their loads overlap
             while (mask) {
-                const uint32_t j1 = (uint32_t)__builtin_ctzll(mask);
+                const uint32_t j1 = (uint32_t)__builtin_ctz(mask);
                 mask &= mask - 1;
                 const bool two = mask != 0;
-                const uint32_t j2 = two ? (uint32_t)__builtin_ctzll(mask) : j1;
+                const uint32_t j2 = two ? (uint32_t)__builtin_ctz(mask) : j1;
                 if (two) mask &= mask - 1;
                 const uint32_t s1 = w0 + j1, s2 = w0 + j2;
                 const float4* r1p = sc.sgeom + 5 * (size_t)(base + s1);
