@@ -499,7 +499,16 @@ __device__ __forceinline__ void leaf(RayState& st, const DevScene& sc, RayDbg* d
                 const float4* r1p = sc.sgeom + 5 * (size_t)(base + s1);
                 const float4* r2p = sc.sgeom + 5 * (size_t)(base + s2);
                 const float4 a1 = r1p[0], b1 = r1p[1], c1 = r1p[2], e1 = r1p[3], f1 = r1p[4];
-                const float4 a2 = r2p[0], b2 = r2p[1], c2 = r2p[2], e2 = r2p[3], f2 = r2p[4];
+                // a lane with one survivor left reads no second record: the
+                // texture path, not the ALU, is the busier unit here
+                float4 a2 = a1, b2 = b1, c2 = c1, e2 = e1, f2 = f1;
+                if (two) {
+                    a2 = r2p[0];
+                    b2 = r2p[1];
+                    c2 = r2p[2];
+                    e2 = r2p[3];
+                    f2 = r2p[4];
+                }
                 const uint32_t g1 = __float_as_uint(f1.w), g2 = __float_as_uint(f2.w);
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
