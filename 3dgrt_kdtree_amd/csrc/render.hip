@@ -441,6 +441,7 @@ struct kdgrt_scene {
     kdgrt_context* ctx = nullptr;
     DevScene dev{};
     void* d_nodes = nullptr;
+    void* d_quads = nullptr;
     void* d_offs = nullptr;
     void* d_sgeom = nullptr;
     void* d_shade = nullptr;
@@ -771,7 +772,7 @@ kdgrt_status kdgrt_framebuffer_to_u8(kdgrt_context* ctx, const float* fb, int wi
 
 uint64_t kdgrt_scene_bytes_estimate(uint32_t n_nodes, uint64_t n_offsets, uint64_t n_gaussians) {
     auto r = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
-    return r(8ull * n_nodes) + r(4ull * n_offsets) + r(80ull * std::max<uint64_t>(1, n_offsets)) +
+    return r(8ull * n_nodes) + r(48ull * n_nodes) + r(4ull * n_offsets) + r(80ull * std::max<uint64_t>(1, n_offsets)) +
            r(192ull * n_gaussians) + 2 * r(16ull * (n_offsets + kLsphPad));
 }
 
@@ -837,6 +838,7 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
     sc->ctx = ctx;
     auto fail = [&](hipError_t e, const char* what) {
         if (sc->d_nodes) hipFree(sc->d_nodes);
+        if (sc->d_quads) hipFree(sc->d_quads);
         if (sc->d_offs) hipFree(sc->d_offs);
         if (sc->d_sgeom) hipFree(sc->d_sgeom);
         if (sc->d_shade) hipFree(sc->d_shade);
@@ -857,6 +859,28 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
     if ((e = hipMemcpy(sc->d_lsph, lsph.data(), 16ull * lsph.size(), hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "copy");
     if ((e = hipMemcpy(sc->d_nodes, nodes, 8ull * n_nodes, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "copy");
+    {
+        // quad records: for the child pair at c (c = first child of an inner
+        // node), nodes c and c+1 and the child pairs of each (zero for a leaf)
+        std::vector<uint4> quads(3 * (size_t)n_nodes, make_uint4(0, 0, 0, 0));
+        auto pair_at = [&](uint32_t c) {
+            return make_uint4(nodes[c].x, nodes[c].y, nodes[c + 1].x, nodes[c + 1].y);
+        };
+        parallel_ranges(n_nodes, [&](uint64_t p0, uint64_t p1) {
+            for (uint64_t p = p0; p < p1; ++p) {
+                if ((nodes[p].x & 7u) == 3u) continue;
+                const uint32_t c = nodes[p].x >> 3;
+                quads[3 * (size_t)c] = pair_at(c);
+                for (int k = 0; k < 2; ++k) {
+                    const kdgrt_node& n = nodes[c + k];
+                    if ((n.x & 7u) != 3u) quads[3 * (size_t)c + 1 + k] = pair_at(n.x >> 3);
+                }
+            }
+        });
+        if ((e = hipMalloc(&sc->d_quads, 16ull * quads.size())) != hipSuccess) return fail(e, "node quads");
+        if ((e = hipMemcpy(sc->d_quads, quads.data(), 16ull * quads.size(), hipMemcpyHostToDevice)) != hipSuccess)
+            return fail(e, "copy");
+    }
     if (n_offsets && (e = hipMemcpy(sc->d_offs, offsets, 4ull * n_offsets, hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "copy");
     {
@@ -877,6 +901,7 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
     if ((e = hipMemcpy(sc->d_shade, shade.data(), 192ull * n_g, hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "copy");
     sc->dev.nodes = (const uint2*)sc->d_nodes;
+    sc->dev.quads = (const uint4*)sc->d_quads;
     sc->dev.offs = (const uint32_t*)sc->d_offs;
     sc->dev.sgeom = (const float4*)sc->d_sgeom;
     sc->dev.shade = (const float4*)sc->d_shade;
@@ -904,6 +929,7 @@ kdgrt_status kdgrt_scene_destroy(kdgrt_scene* sc) {
     if (!sc) return KDGRT_OK;
     hipSetDevice(sc->ctx->device);
     hipFree(sc->d_nodes);
+    hipFree(sc->d_quads);
     hipFree(sc->d_offs);
     hipFree(sc->d_sgeom);
     hipFree(sc->d_shade);

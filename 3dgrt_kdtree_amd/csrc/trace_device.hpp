@@ -51,6 +51,7 @@ namespace kdgrt_dev {
 
 struct DevScene {
     const uint2* __restrict__ nodes;
+    const uint4* __restrict__ quads;  // per child pair c: nodes c, c+1, then the child pairs of both (48 B)
     const uint32_t* __restrict__ offs;
     const float4* __restrict__ shade; // 12 x float4 per Gaussian, channel-major (f_dc[c], f_rest[15c..15c+14])
     const float4* __restrict__ lsph;  // bound spheres in leaf-slot order: centre, packed (radius, kappa) as 2 x f16
@@ -377,30 +378,45 @@ __device__ __forceinline__ bool ray_begin(RayState& st, const DevScene& sc, cons
 // stk[64 (depth + e)], t_far at stk[64 (2 depth + e)] (32-bit words, entry-
 // major over the wave's lanes).  An entry holds the pushed node itself, so a
 // pop needs no global load.
+// One inner-node step (cudaRenderer.cu:1555-1573) with the node's children
+// n0, n1 at hand; returns which child is next (0 near-side n0, 1 n1).
+__device__ __forceinline__ uint32_t node_step(RayState& st, const uint2 n0, const uint2 n1, int depth,
+                                              uint32_t* __restrict__ stk) {
+    const uint32_t axis = st.node.x & 3u;
+    const float oa = sel_axis(axis, st.ox, st.oy, st.oz);
+    const float ia = sel_axis(axis, st.ix, st.iy, st.iz);
+    const float t_split = (__uint_as_float(st.node.y) - oa) * ia;
+    const uint32_t sign = (st.signs >> (axis > 2u ? 2u : axis)) & 1u;
+    const uint32_t pick = sign ^ (t_split <= st.t_near ? 1u : 0u);
+    if (st.t_near < t_split && t_split < st.t_far) {
+        if (++st.top == (uint32_t)depth) st.top = 0; // push overwrites the oldest when full
+        st.quant = st.quant + 1 < (uint32_t)depth ? st.quant + 1 : (uint32_t)depth;
+        const uint2 far = sign ? n0 : n1; // child + (sign ^ 1)
+        stk[64 * st.top] = far.x;
+        stk[64 * (depth + st.top)] = far.y;
+        stk[64 * (2 * depth + st.top)] = __float_as_uint(st.t_far);
+        st.t_far = t_split;
+    }
+    st.node = pick ? n1 : n0;
+    return pick;
+}
+
+// Descent to the next leaf, two levels per memory round trip: the 48-B quad
+// record of a child pair holds the pair and both of its child pairs, so the
+// second step needs no load of its own.  The steps, pushes and counters are
+// those of the one-level loop.
 template <bool DBG>
 __device__ __forceinline__ void descend(RayState& st, const DevScene& sc, int depth, uint32_t* __restrict__ stk,
                                         RayDbg* dbg) {
     while ((st.node.x & 7u) != 3u) {
-        const uint32_t child = st.node.x >> 3;
-        const uint32_t axis = st.node.x & 3u;
-        // both children (pair-allocated at child, child + 1) in one step
-        const uint2 n0 = sc.nodes[child], n1 = sc.nodes[child + 1];
-        const float oa = sel_axis(axis, st.ox, st.oy, st.oz);
-        const float ia = sel_axis(axis, st.ix, st.iy, st.iz);
-        const float t_split = (__uint_as_float(st.node.y) - oa) * ia;
-        const uint32_t sign = (st.signs >> (axis > 2u ? 2u : axis)) & 1u;
-        const uint32_t pick = sign ^ (t_split <= st.t_near ? 1u : 0u);
-        if (st.t_near < t_split && t_split < st.t_far) {
-            if (++st.top == (uint32_t)depth) st.top = 0; // push overwrites the oldest when full
-            st.quant = st.quant + 1 < (uint32_t)depth ? st.quant + 1 : (uint32_t)depth;
-            const uint2 far = sign ? n0 : n1; // child + (sign ^ 1)
-            stk[64 * st.top] = far.x;
-            stk[64 * (depth + st.top)] = far.y;
-            stk[64 * (2 * depth + st.top)] = __float_as_uint(st.t_far);
-            st.t_far = t_split;
-        }
+        const uint4* qp = sc.quads + 3 * (size_t)(st.node.x >> 3);
+        const uint4 q0 = qp[0], q1 = qp[1], q2 = qp[2];
+        const uint32_t pick = node_step(st, make_uint2(q0.x, q0.y), make_uint2(q0.z, q0.w), depth, stk);
         if (DBG) dbg->cnt[0]++;
-        st.node = pick ? n1 : n0;
+        if ((st.node.x & 7u) == 3u) break;
+        const uint4 g = pick ? q2 : q1;
+        node_step(st, make_uint2(g.x, g.y), make_uint2(g.z, g.w), depth, stk);
+        if (DBG) dbg->cnt[0]++;
     }
 }
 
