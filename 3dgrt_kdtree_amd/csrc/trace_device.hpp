@@ -483,22 +483,49 @@ __device__ __forceinline__ void leaf(RayState& st, const DevScene& sc, RayDbg* d
             // four slots per step, one contiguous read; the group may run past
             // the leaf (the record arrays are padded) and bits at or above wn
             // are cleared afterwards
-            for (uint32_t j0 = 0; j0 < wn; j0 += 4) {
-                uint32_t bits;
-                if (CONE || (HYB && st.cone)) {
-                    const float4* cp = sc.vcone + base + w0 + j0;
-                    const float4 c0 = cp[0], c1 = cp[1], c2 = cp[2], c3 = cp[3];
-                    bits = (cone_pass(d, c0) ? 1u : 0u) | (cone_pass(d, c1) ? 2u : 0u) |
-                           (cone_pass(d, c2) ? 4u : 0u) | (cone_pass(d, c3) ? 8u : 0u);
-                } else {
-                    const float4* sp = sc.lsph + base + w0 + j0;
-                    const float4 s0 = sp[0], s1 = sp[1], s2 = sp[2], s3 = sp[3];
-                    bits = (sphere_pass(o, d, s0, st.t_near, st.t_far) ? 1u : 0u) |
-                           (sphere_pass(o, d, s1, st.t_near, st.t_far) ? 2u : 0u) |
-                           (sphere_pass(o, d, s2, st.t_near, st.t_far) ? 4u : 0u) |
-                           (sphere_pass(o, d, s3, st.t_near, st.t_far) ? 8u : 0u);
+            if (!CONE && !HYB) {
+                // sphere pre-test only (ray lists, batches): the next group's
+                // records are requested before this group is tested, which
+                // hides their latency behind the test's arithmetic
+                const float4* rp = sc.lsph + base + w0;
+                float4 r0 = rp[0], r1 = rp[1], r2 = rp[2], r3 = rp[3];
+                for (uint32_t j0 = 0; j0 < wn; j0 += 4) {
+                    float4 n0 = r0, n1 = r1, n2 = r2, n3 = r3;
+                    if (j0 + 4 < wn) {
+                        const float4* np = rp + j0 + 4;
+                        n0 = np[0];
+                        n1 = np[1];
+                        n2 = np[2];
+                        n3 = np[3];
+                    }
+                    const uint32_t bits = (sphere_pass(o, d, r0, st.t_near, st.t_far) ? 1u : 0u) |
+                                          (sphere_pass(o, d, r1, st.t_near, st.t_far) ? 2u : 0u) |
+                                          (sphere_pass(o, d, r2, st.t_near, st.t_far) ? 4u : 0u) |
+                                          (sphere_pass(o, d, r3, st.t_near, st.t_far) ? 8u : 0u);
+                    mask |= bits << j0;
+                    r0 = n0;
+                    r1 = n1;
+                    r2 = n2;
+                    r3 = n3;
                 }
-                mask |= bits << j0;
+            } else {
+                for (uint32_t j0 = 0; j0 < wn; j0 += 4) {
+                    uint32_t bits;
+                    if (CONE || (HYB && st.cone)) {
+                        const float4* cp = sc.vcone + base + w0 + j0;
+                        const float4 c0 = cp[0], c1 = cp[1], c2 = cp[2], c3 = cp[3];
+                        bits = (cone_pass(d, c0) ? 1u : 0u) | (cone_pass(d, c1) ? 2u : 0u) |
+                               (cone_pass(d, c2) ? 4u : 0u) | (cone_pass(d, c3) ? 8u : 0u);
+                    } else {
+                        const float4* sp = sc.lsph + base + w0 + j0;
+                        const float4 s0 = sp[0], s1 = sp[1], s2 = sp[2], s3 = sp[3];
+                        bits = (sphere_pass(o, d, s0, st.t_near, st.t_far) ? 1u : 0u) |
+                               (sphere_pass(o, d, s1, st.t_near, st.t_far) ? 2u : 0u) |
+                               (sphere_pass(o, d, s2, st.t_near, st.t_far) ? 4u : 0u) |
+                               (sphere_pass(o, d, s3, st.t_near, st.t_far) ? 8u : 0u);
+                    }
+                    mask |= bits << j0;
+                }
             }
             if (wn < 32) mask &= (1u << wn) - 1u;
             if (PROF) dbg->pretests += wn;
