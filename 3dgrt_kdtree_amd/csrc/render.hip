@@ -53,6 +53,7 @@ struct ViewArgs {
     int W, H;
     int depth;
     int blocks_x, blocks_y;
+    int tile4; // 4x4-pixel tiles, 16 rays per wave (views too small to fill the chip)
     void* out;
     int32_t* counters;
     uint64_t* hashes;
@@ -87,8 +88,10 @@ __global__ __launch_bounds__(64) void k_view(ViewArgs a) {
     const int tx = (int)(st % (uint32_t)a.blocks_x) * 8 + (int)(local % 8u);
     const int ty = (int)(st / (uint32_t)a.blocks_x) * 8 + (int)(local / 8u);
     const int lane = threadIdx.x;
-    const int x = tx * 8 + (lane & 7);
-    const int y = ty * 8 + (lane >> 3);
+    const int sh = a.tile4 ? 2 : 3; // log2 of the tile edge
+    if (a.tile4 && lane >= 16) return;
+    const int x = (tx << sh) + (lane & ((1 << sh) - 1));
+    const int y = (ty << sh) + (lane >> sh);
     if (x >= a.W || y >= a.H) return;
     float o[3], d[3], rgb[3], A;
     make_ray(a.cam, x, y, o, d);
@@ -193,8 +196,10 @@ __global__ __launch_bounds__(64) void k_hybrid(HybridArgs ha) {
     const int tx = (int)(st % (uint32_t)a.blocks_x) * 8 + (int)(local % 8u);
     const int ty = (int)(st / (uint32_t)a.blocks_x) * 8 + (int)(local / 8u);
     const int lane = threadIdx.x;
-    const int x = tx * 8 + (lane & 7);
-    const int y = ty * 8 + (lane >> 3);
+    const int sh = a.tile4 ? 2 : 3; // log2 of the tile edge
+    if (a.tile4 && lane >= 16) return;
+    const int x = (tx << sh) + (lane & ((1 << sh) - 1));
+    const int y = (ty << sh) + (lane >> sh);
     if (x >= a.W || y >= a.H) return;
     float o[3], d[3];
     make_ray(a.cam, x, y, o, d);
@@ -965,6 +970,17 @@ static void fill_view_args(ViewArgs& a, const kdgrt_scene* sc, const kdgrt_camer
     a.depth = depth;
     a.blocks_x = (W + 63) / 64; // 64x64-pixel groups of 64 8x8 tiles
     a.blocks_y = (H + 63) / 64;
+    // A view with fewer 8x8 tiles than half the chip's wave slots leaves the
+    // chip idle and its time is that of its slowest tile: 4x4 tiles (16 rays
+    // per wave, four times the waves) shorten the slowest tile instead
+    // (320x240: 1.09 -> 0.85 ms).  Groups of 64 tiles are then 32x32 px.
+    const uint64_t slots = (uint64_t)std::max(1, sc->ctx->prop.multiProcessorCount) * 16u;
+    a.tile4 = 0;
+    if ((uint64_t)((W + 7) / 8) * (uint64_t)((H + 7) / 8) * 2u < slots) {
+        a.tile4 = 1;
+        a.blocks_x = (W + 31) / 32;
+        a.blocks_y = (H + 31) / 32;
+    }
     a.out = out;
     a.counters = nullptr;
     a.hashes = nullptr;
