@@ -532,6 +532,12 @@ class Scene:
         check(lib().kdgrt_scene_device_bytes(self.h, C.byref(b)))
         return b.value
 
+    def set_tile_schedule(self, enable):
+        """Frame-to-frame tile order of Device.render (on by default): later
+        launches of a nearly unchanged view start the tiles that ran longest
+        first.  Results are the same either way."""
+        check(lib().kdgrt_scene_set_tile_schedule(self.h, C.c_int(1 if enable else 0)))
+
     def free(self):
         if self.h:
             lib().kdgrt_scene_destroy(self.h)

@@ -54,6 +54,8 @@ struct ViewArgs {
     int depth;
     int blocks_x, blocks_y;
     int tile4; // 4x4-pixel tiles, 16 rays per wave (views too small to fill the chip)
+    const uint32_t* order; // launch order of the logical tiles (null: XCD-interleaved raster order)
+    uint32_t* cost;        // per logical tile: duration of this launch in wall-clock ticks (null: not recorded)
     void* out;
     int32_t* counters;
     uint64_t* hashes;
@@ -81,14 +83,19 @@ template <int K, bool DBG, int FMT, bool PROF = false, bool CONE = true>
 __global__ __launch_bounds__(64) void k_view(ViewArgs a) {
     // One wavefront per block = one 8x8-pixel tile, so a finished wave frees
     // its slot at once (per-ray cost varies widely across an image).  Groups
-    // of 64 tiles (64x64 px) share an XCD.
-    const uint32_t L = xcd_interleave<64>(blockIdx.x);
+    // of 64 tiles (64x64 px) share an XCD, unless the launch follows a tile
+    // order (longest tiles of the previous launch first).
+    const uint32_t L = a.order ? a.order[blockIdx.x] : xcd_interleave<64>(blockIdx.x);
+    const uint64_t t_start = wall_clock64();
     const uint32_t st = L / 64u, local = L % 64u;
-    if (st >= (uint32_t)a.blocks_x * (uint32_t)a.blocks_y) return;
     const int tx = (int)(st % (uint32_t)a.blocks_x) * 8 + (int)(local % 8u);
     const int ty = (int)(st / (uint32_t)a.blocks_x) * 8 + (int)(local / 8u);
     const int lane = threadIdx.x;
     const int sh = a.tile4 ? 2 : 3; // log2 of the tile edge
+    if (st >= (uint32_t)a.blocks_x * (uint32_t)a.blocks_y || (tx << sh) >= a.W || (ty << sh) >= a.H) {
+        if (a.cost && lane == 0) a.cost[L] = 0; // a logical tile outside the image
+        return;
+    }
     if (a.tile4 && lane >= 16) return;
     const int x = (tx << sh) + (lane & ((1 << sh) - 1));
     const int y = (ty << sh) + (lane >> sh);
@@ -131,6 +138,72 @@ __global__ __launch_bounds__(64) void k_view(ViewArgs a) {
             for (int i = 0; i < 8; ++i) a.counters[8 * rid + i] = dbg.cnt[i];
         if (a.hashes) a.hashes[rid] = dbg.hash;
     }
+    // lane 0 holds the tile's first pixel, which is inside the image
+    if (a.cost && lane == 0) {
+        const uint64_t dt = (uint64_t)(wall_clock64() - (long long)t_start);
+        a.cost[L] = dt < 0xFFFFFFFFull ? (uint32_t)dt : 0xFFFFFFFFu;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tile order of the next launch of a view (frame-to-frame schedule).
+// ---------------------------------------------------------------------------
+// A launch lasts at least as long as its slowest tile after that tile
+// starts, so tiles that start late and run long leave the chip idle at the
+// end (a 1080p view of the 1M-Gaussian scene keeps 77 % of its wave slots
+// busy in raster order).  k_view records the duration of every tile; this
+// single-workgroup counting sort turns the smoothed durations (mean of the
+// last value and this one, the plain value after a camera change) into a
+// longest-first order for the next launch of the same view.  Durations fall
+// into 512 log-spaced classes, 16 per octave; order within a class is that of
+// the LDS atomics.  Any duration values give a permutation of [0, n).
+constexpr int kSchedClasses = 512;
+constexpr uint32_t kSchedMaxTiles = 1u << 20;
+
+__device__ __forceinline__ uint32_t sched_class(uint32_t c) {
+    if (c == 0) return 0;
+    const uint32_t e = 31u - (uint32_t)__builtin_clz(c);
+    const uint32_t m = e >= 4u ? (c >> (e - 4u)) & 15u : (c << (4u - e)) & 15u;
+    return e * 16u + m;
+}
+
+__global__ __launch_bounds__(1024) void k_tile_sched(const uint32_t* __restrict__ cost, uint32_t* __restrict__ smooth,
+                                                     uint32_t* __restrict__ order, uint32_t n, int reset) {
+    __shared__ uint32_t cls[kSchedClasses];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t k = tid; k < (uint32_t)kSchedClasses; k += blockDim.x) cls[k] = 0;
+    __syncthreads();
+    for (uint32_t t = tid; t < n; t += blockDim.x) {
+        const uint32_t c = cost[t], e = smooth[t];
+        const uint32_t v = (reset || e == 0) ? c : (uint32_t)(((uint64_t)e + c + 1u) >> 1);
+        smooth[t] = v;
+        atomicAdd(&cls[sched_class(v)], 1u);
+    }
+    __syncthreads();
+    if (tid < 64) {
+        // exclusive scan from the top class down: lane l owns the classes
+        // 511 - 8l ... 504 - 8l
+        uint32_t cnt[8], sum = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            cnt[i] = cls[kSchedClasses - 1 - (8 * (int)tid + i)];
+            sum += cnt[i];
+        }
+        uint32_t incl = sum;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t v = __shfl_up(incl, d);
+            if ((int)tid >= d) incl += v;
+        }
+        uint32_t run = incl - sum;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            cls[kSchedClasses - 1 - (8 * (int)tid + i)] = run;
+            run += cnt[i];
+        }
+    }
+    __syncthreads();
+    for (uint32_t t = tid; t < n; t += blockDim.x) order[atomicAdd(&cls[sched_class(smooth[t])], 1u)] = t;
 }
 
 // ---------------------------------------------------------------------------
@@ -459,6 +532,17 @@ struct kdgrt_scene {
     hipStream_t cone_stream = nullptr;
     bool cone_used = false;
     std::mutex cone_mu;
+    // Frame-to-frame tile schedule of kdgrt_render (guarded by cone_mu and
+    // ordered by cone_ev like the cone table): d_sched holds sched_cap
+    // durations, sched_cap smoothed durations and sched_cap tile indices (the
+    // order for the next launch of the view in sched_cam / sched_w x sched_h /
+    // sched_depth).
+    uint32_t* d_sched = nullptr;
+    uint32_t sched_cap = 0;
+    bool sched_on = true;
+    bool sched_valid = false;
+    int sched_w = 0, sched_h = 0, sched_depth = 0;
+    CamInfo sched_cam{};
     uint64_t bytes = 0;
     uint32_t n_nodes = 0;
     uint64_t n_offsets = 0, n_gaussians = 0;
@@ -940,8 +1024,17 @@ kdgrt_status kdgrt_scene_destroy(kdgrt_scene* sc) {
     hipFree(sc->d_shade);
     hipFree(sc->d_lsph);
     hipFree(sc->d_vcone);
+    hipFree(sc->d_sched);
     hipEventDestroy(sc->cone_ev);
     delete sc;
+    return KDGRT_OK;
+}
+
+kdgrt_status kdgrt_scene_set_tile_schedule(kdgrt_scene* sc, int enable) {
+    if (!sc) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "scene_set_tile_schedule: null scene");
+    std::lock_guard<std::mutex> lock(sc->cone_mu);
+    sc->sched_on = enable != 0;
+    sc->sched_valid = false;
     return KDGRT_OK;
 }
 
@@ -976,6 +1069,8 @@ static void fill_view_args(ViewArgs& a, const kdgrt_scene* sc, const kdgrt_camer
     // (320x240: 1.09 -> 0.85 ms).  Groups of 64 tiles are then 32x32 px.
     const uint64_t slots = (uint64_t)std::max(1, sc->ctx->prop.multiProcessorCount) * 16u;
     a.tile4 = 0;
+    a.order = nullptr;
+    a.cost = nullptr;
     if ((uint64_t)((W + 7) / 8) * (uint64_t)((H + 7) / 8) * 2u < slots) {
         a.tile4 = 1;
         a.blocks_x = (W + 31) / 32;
@@ -1003,7 +1098,8 @@ static kdgrt_status with_cone_table(const kdgrt_scene* csc, ViewArgs& a, hipStre
     hipLaunchKernelGGL(k_cone_prep, dim3(prep_grid), dim3(256), 0, s, (const float4*)sc->d_lsph, n, a.cam.eye[0],
                        a.cam.eye[1], a.cam.eye[2], (float4*)sc->d_vcone);
     a.sc.vcone = (const float4*)sc->d_vcone;
-    launch();
+    const kdgrt_status st = launch(sc);
+    if (st != KDGRT_OK) return st;
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(sc->cone_ev, s));
     sc->cone_stream = s;
@@ -1012,11 +1108,90 @@ static kdgrt_status with_cone_table(const kdgrt_scene* csc, ViewArgs& a, hipStre
 }
 } // extern "C++"
 
+// Whether the previous view of the scene is close enough to this one for its
+// tile durations to order this launch: same size and stack depth, image axes
+// and centre ray within half a degree, eye moved by less than 1 % of its
+// distance to the scene box centre, pixel footprint within 1 %.  (Orbiting
+// the 1M-Gaussian scene by 0.25 degrees per frame keeps about half of the
+// static view's gain; at 1 degree per frame none is left.)
+static bool sched_similar(const kdgrt_scene* sc, const ViewArgs& a) {
+    if (!sc->sched_valid || sc->sched_w != a.W || sc->sched_h != a.H || sc->sched_depth != a.depth) return false;
+    const CamInfo& p = sc->sched_cam;
+    const CamInfo& c = a.cam;
+    auto centre_ray = [&](const CamInfo& k, double r[3]) {
+        for (int i = 0; i < 3; ++i)
+            r[i] = (double)k.start[i] + (double)k.u[i] * k.step_x * (0.5 * a.W) -
+                   (double)k.v[i] * k.step_y * (0.5 * a.H) - (double)k.eye[i];
+    };
+    auto cos_angle = [](const double* x, const double* y) {
+        double xy = 0, xx = 0, yy = 0;
+        for (int i = 0; i < 3; ++i) {
+            xy += x[i] * y[i];
+            xx += x[i] * x[i];
+            yy += y[i] * y[i];
+        }
+        return (xx > 0 && yy > 0) ? xy / std::sqrt(xx * yy) : -1.0;
+    };
+    const double cos_lim = 0.99996192306417; // cos(0.5 degrees)
+    double r0[3], r1[3], u0[3], u1[3], v0[3], v1[3], de2 = 0, dist2 = 0;
+    centre_ray(p, r0);
+    centre_ray(c, r1);
+    for (int i = 0; i < 3; ++i) {
+        u0[i] = p.u[i];
+        u1[i] = c.u[i];
+        v0[i] = p.v[i];
+        v1[i] = c.v[i];
+        const double de = (double)c.eye[i] - (double)p.eye[i];
+        const double dc = 0.5 * ((double)sc->dev.bmin[i] + (double)sc->dev.bmax[i]) - (double)c.eye[i];
+        de2 += de * de;
+        dist2 += dc * dc;
+    }
+    if (cos_angle(r0, r1) < cos_lim || cos_angle(u0, u1) < cos_lim || cos_angle(v0, v1) < cos_lim) return false;
+    if (de2 > 1e-4 * dist2) return false;
+    auto footprint = [](const float* w, float step) {
+        return std::sqrt((double)w[0] * w[0] + (double)w[1] * w[1] + (double)w[2] * w[2]) * std::fabs((double)step);
+    };
+    const double fx0 = footprint(p.u, p.step_x), fx1 = footprint(c.u, c.step_x);
+    const double fy0 = footprint(p.v, p.step_y), fy1 = footprint(c.v, c.step_y);
+    return std::fabs(fx1 - fx0) <= 0.01 * fx0 && std::fabs(fy1 - fy0) <= 0.01 * fy0;
+}
+
 typedef void (*ViewKernel)(ViewArgs);
-static kdgrt_status launch_view(const kdgrt_scene* csc, ViewArgs& a, ViewKernel kern, hipStream_t s) {
-    return with_cone_table(csc, a, s, [&] {
-        const dim3 grid(interleave_grid<64>((uint64_t)a.blocks_x * a.blocks_y * 64)), block(64);
-        hipLaunchKernelGGL(kern, grid, block, stack_lds_bytes(a.depth, 64), s, a);
+// sched: record tile durations and follow the scene's tile schedule (the
+// plain kernel only; debug and profile launches keep the raster order).
+static kdgrt_status launch_view(const kdgrt_scene* csc, ViewArgs& a, ViewKernel kern, hipStream_t s,
+                                bool sched = false) {
+    return with_cone_table(csc, a, s, [&](kdgrt_scene* sc) -> kdgrt_status {
+        const uint32_t n = interleave_grid<64>((uint64_t)a.blocks_x * a.blocks_y * 64);
+        const bool use = sched && sc->sched_on && n <= kSchedMaxTiles;
+        bool similar = false;
+        if (use) {
+            if (n > sc->sched_cap) {
+                // earlier launches of this scene (ordered by cone_ev) may
+                // still read the old buffer
+                if (sc->cone_used) HIP_TRY(hipEventSynchronize(sc->cone_ev));
+                HIP_TRY(hipFree(sc->d_sched));
+                sc->d_sched = nullptr;
+                sc->sched_cap = 0;
+                sc->sched_valid = false;
+                HIP_TRY(hipMalloc((void**)&sc->d_sched, 3ull * n * sizeof(uint32_t)));
+                sc->sched_cap = n;
+            }
+            similar = sched_similar(sc, a);
+            a.cost = sc->d_sched;
+            a.order = similar ? sc->d_sched + 2ull * sc->sched_cap : nullptr;
+        }
+        hipLaunchKernelGGL(kern, dim3(n), dim3(64), stack_lds_bytes(a.depth, 64), s, a);
+        if (use) {
+            hipLaunchKernelGGL(k_tile_sched, dim3(1), dim3(1024), 0, s, sc->d_sched, sc->d_sched + sc->sched_cap,
+                               sc->d_sched + 2ull * sc->sched_cap, n, similar ? 0 : 1);
+            sc->sched_valid = true;
+            sc->sched_w = a.W;
+            sc->sched_h = a.H;
+            sc->sched_depth = a.depth;
+            sc->sched_cam = a.cam;
+        }
+        return KDGRT_OK;
     });
 }
 
@@ -1031,7 +1206,7 @@ kdgrt_status kdgrt_render(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgrt
     return launch_view(sc, a,
                        fmt == KDGRT_OUT_RGBT_F32 ? k_view<kViewK, false, KDGRT_OUT_RGBT_F32>
                                                  : k_view<kViewK, false, KDGRT_OUT_RGB_F32>,
-                       (hipStream_t)stream);
+                       (hipStream_t)stream, true);
 }
 
 kdgrt_status kdgrt_render_debug(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgrt_camera_info* cam, int W,
@@ -1100,9 +1275,10 @@ kdgrt_status kdgrt_render_hybrid(kdgrt_context* ctx, const kdgrt_scene* sc, cons
     }
     fill_view_args(ha.v, sc, cam, W, H, depth, out);
     const hipStream_t s = (hipStream_t)stream;
-    return with_cone_table(sc, ha.v, s, [&] {
+    return with_cone_table(sc, ha.v, s, [&](kdgrt_scene*) -> kdgrt_status {
         const dim3 grid(interleave_grid<64>((uint64_t)ha.v.blocks_x * ha.v.blocks_y * 64)), block(64);
         hipLaunchKernelGGL(k_hybrid<kViewK>, grid, block, stack_lds_bytes(depth, 64), s, ha);
+        return KDGRT_OK;
     });
 }
 

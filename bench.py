@@ -321,6 +321,45 @@ def hybrid_bench(dev, sc, ci, aabb, W, H, stream, steps, warmup):
     return r
 
 
+def orbit_view(aabb, yaw_deg, w=1920, h=1080):
+    """view_1080p's camera turned by yaw_deg about the vertical through the
+    box centre."""
+    centre = 0.5 * (aabb[:3].astype(np.float64) + aabb[3:].astype(np.float64))
+    d = np.array([0.48, 0.36, 0.8])
+    d /= np.linalg.norm(d)
+    a = np.radians(yaw_deg)
+    d = np.array([d[0] * np.cos(a) + d[2] * np.sin(a), d[1], -d[0] * np.sin(a) + d[2] * np.cos(a)])
+    cam = K.look_at(centre + 25.0 * d, centre, 60.0, w / h, 0.005)
+    return K.camera_make(cam, w, h)
+
+
+def tile_order_legs(dev, sc, ci, aabb, W, H, out, stream, steps):
+    """The plain view in raster tile order (schedule off), and a camera that
+    orbits by 0.1 and 0.25 degrees per launch with and without the schedule:
+    how much of the repeated view's gain survives motion."""
+    def timed(cams, sched):
+        sc.set_tile_schedule(sched)
+        dev.render(sc, cams[0], W, H, out, stream=stream)
+        dev.sync()
+        dev.timer_begin(stream)
+        for c in cams:
+            dev.render(sc, c, W, H, out, stream=stream)
+        ms = dev.timer_end(stream) / len(cams)
+        dev.sync()
+        return ms
+    r = {}
+    ms = timed([ci] * steps, False)
+    r["raster_order"] = {"ms_per_launch": round(ms, 4), "mrays_s": round(W * H / (ms * 1e-3) / 1e6, 1)}
+    for step in (0.1, 0.25):
+        cams = [orbit_view(aabb, step * k, W, H) for k in range(steps)]
+        m_sched, m_raster = timed(cams, True), timed(cams, False)
+        r[f"orbit_{step}_deg_per_launch"] = {"scheduled_mrays_s": round(W * H / (m_sched * 1e-3) / 1e6, 1),
+                                             "raster_mrays_s": round(W * H / (m_raster * 1e-3) / 1e6, 1)}
+    sc.set_tile_schedule(True)
+    log(f"tile order: {r}")
+    return r
+
+
 def cpu_baseline(tree, g_inv, ci, w, h, budget_s):
     import oracle_lib as O
     threads = O.lib().oracle_max_threads()
@@ -348,6 +387,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-multi", action="store_true", help="skip the multi-scene batch (256 views + 4M rays)")
     ap.add_argument("--no-hybrid", action="store_true", help="skip the hybrid (secondary-sphere) view")
+    ap.add_argument("--no-order", action="store_true",
+                    help="skip the raster-order and moving-camera legs of the plain view")
     ap.add_argument("--serve-scenes", type=int, default=16)
     ap.add_argument("--serve-window-ms", type=float, default=0.0,
                     help="how long a busy device waits to fill a batch (0: launch as soon as a slot is free)")
@@ -423,12 +464,18 @@ def main():
               "warmup": args.warmup, "ms_per_step": round(ms_launch, 4), "higher_is_better": True, "scaling": "weak",
               "vs_baseline": None, "dtype": "f32", "data": "synthetic",
               "config": {"workload": f"plain one-view launch: {len(g)} clustered Gaussians, one 1920x1080 pinhole "
-                                     f"view from radius 25, 1 ray/pixel, short stack 12",
-                         "gaussians": len(g), "width": W, "height": H, "stack_depth": 12},
+                                     f"view from radius 25, 1 ray/pixel, short stack 12, the view launched "
+                                     f"repeatedly (the reference's render-test loop), tiles ordered by the "
+                                     f"previous launch's tile durations",
+                         "gaussians": len(g), "width": W, "height": H, "stack_depth": 12,
+                         "tile_order": "frame-to-frame schedule (raster-order rate under tile_order)"},
               "device": info["name"], "roofline": roof, "hbm_copy": copy,
               **({"phases": phases} if phases else {}),
               "build": {"host_sah_build_s": round(build_s, 2), "threads": os.cpu_count(), "nodes": st["n_nodes"],
                         "leaf_refs": st["n_offsets"], "proxy_triangles": st["n_triangles"]}}
+
+    if not args.no_order:
+        result["tile_order"] = tile_order_legs(dev, sc, ci, aabb, W, H, out, stream, args.steps)
 
     if not args.no_hybrid:
         result["hybrid"] = hybrid_bench(dev, sc, ci, aabb, W, H, stream, args.steps, args.warmup)

@@ -1,0 +1,89 @@
+"""Frame-to-frame tile schedule of the plain view launch (kdgrt_render).
+
+The launch order of the tiles changes no pixel: frames rendered with the
+schedule on (later launches of a nearly unchanged view take the tiles that
+ran longest first) equal frames rendered in raster order bit for bit, over
+repeated views and small camera moves (the order is followed), a camera jump
+and size changes (it is not), views small enough for 4x4-pixel tiles, and
+the RGB output format.  Output buffers are filled with NaN before every launch
+so that a tile left out by an order would show.
+"""
+import importlib
+
+import numpy as np
+import pytest
+
+import scenes as S
+
+K = importlib.import_module("3dgrt_kdtree_amd")
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = K.Device(0)
+    yield d
+    d.close()
+
+
+def _frame(dev, sc, ci, w, h, fmt=K.OUT_RGBT):
+    ch = 4 if fmt == K.OUT_RGBT else 3
+    buf = dev.upload(np.full((h, w, ch), np.nan, np.float32))
+    dev.render(sc, ci, w, h, buf, fmt=fmt)
+    dev.sync()
+    img = buf.download(np.float32, (h, w, ch))
+    buf.free()
+    return img
+
+
+def _orbit(aabb, w, h, yaw_deg):
+    a = np.radians(yaw_deg)
+    d = np.array([0.48, 0.36, 0.8])
+    d = np.array([d[0] * np.cos(a) + d[2] * np.sin(a), d[1], -d[0] * np.sin(a) + d[2] * np.cos(a)])
+    return S.cam_orbit(aabb, w, h, direction=d)
+
+
+def test_schedule_changes_no_pixel(dev):
+    hs = S.clustered(100_000, seed=1011)
+    on = dev.scene(hs.tree, hs.g)
+    off = dev.scene(hs.tree, hs.g)
+    off.set_tile_schedule(False)
+    aabb = hs.aabb
+    seq = [(0.0, 640, 480)] * 3 + [(0.2, 640, 480), (0.35, 640, 480)]  # repeats, then moves within 0.5 degrees
+    seq += [(40.0, 640, 480), (40.0, 640, 480)]                        # a jump, then a repeat
+    seq += [(40.0, 1280, 720)] * 2 + [(40.0, 160, 120)] * 3            # size changes; 160x120 runs as 4x4 tiles
+    seq += [(0.0, 640, 480)] * 2 + [(0.0, 1001, 333)] * 2               # back; a size with partial tiles
+    for k, (yaw, w, h) in enumerate(seq):
+        ci = _orbit(aabb, w, h, yaw)
+        print(f"[schedule] frame {k}: yaw {yaw} deg, {w}x{h}", flush=True)
+        a = _frame(dev, on, ci, w, h)
+        b = _frame(dev, off, ci, w, h)
+        assert not np.isnan(a).any(), f"frame {k}: pixels left unwritten"
+        assert np.array_equal(a, b), f"frame {k}: scheduled and raster frames differ"
+    ci = _orbit(aabb, 640, 480, 0.0)
+    for _ in range(2):
+        assert np.array_equal(_frame(dev, on, ci, 640, 480, K.OUT_RGB), _frame(dev, off, ci, 640, 480, K.OUT_RGB))
+    # the debug kernel (raster order, counters) writes the same image
+    assert np.array_equal(_frame(dev, on, ci, 640, 480), dev.render_debug(on, ci, 640, 480)["rgbt"])
+    on.free()
+    off.free()
+
+
+def test_schedule_toggle_and_repeat_full_size(dev):
+    """The 1M-Gaussian 1080p view, five launches with the schedule: all equal
+    to the raster launch; turning the schedule off and on again keeps it so."""
+    hs = S.clustered(1_000_000)
+    sc = dev.scene(hs.tree, hs.g)
+    ci = S.cam_orbit(hs.aabb, 1920, 1080)
+    sc.set_tile_schedule(False)
+    ref = _frame(dev, sc, ci, 1920, 1080)
+    sc.set_tile_schedule(True)
+    for k in range(5):
+        print(f"[schedule 1M] launch {k}", flush=True)
+        assert np.array_equal(_frame(dev, sc, ci, 1920, 1080), ref), f"launch {k} differs"
+    sc.set_tile_schedule(False)
+    assert np.array_equal(_frame(dev, sc, ci, 1920, 1080), ref)
+    sc.set_tile_schedule(True)
+    assert np.array_equal(_frame(dev, sc, ci, 1920, 1080), ref)
+    sc.free()
