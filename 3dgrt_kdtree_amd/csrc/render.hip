@@ -263,13 +263,17 @@ struct HybridArgs {
 template <int K>
 __global__ __launch_bounds__(64) void k_hybrid(HybridArgs ha) {
     const ViewArgs& a = ha.v;
-    const uint32_t L = xcd_interleave<64>(blockIdx.x);
+    const uint32_t L = a.order ? a.order[blockIdx.x] : xcd_interleave<64>(blockIdx.x);
+    const uint64_t t_start = wall_clock64();
     const uint32_t st = L / 64u, local = L % 64u;
-    if (st >= (uint32_t)a.blocks_x * (uint32_t)a.blocks_y) return;
     const int tx = (int)(st % (uint32_t)a.blocks_x) * 8 + (int)(local % 8u);
     const int ty = (int)(st / (uint32_t)a.blocks_x) * 8 + (int)(local / 8u);
     const int lane = threadIdx.x;
     const int sh = a.tile4 ? 2 : 3; // log2 of the tile edge
+    if (st >= (uint32_t)a.blocks_x * (uint32_t)a.blocks_y || (tx << sh) >= a.W || (ty << sh) >= a.H) {
+        if (a.cost && lane == 0) a.cost[L] = 0; // a logical tile outside the image
+        return;
+    }
     if (a.tile4 && lane >= 16) return;
     const int x = (tx << sh) + (lane & ((1 << sh) - 1));
     const int y = (ty << sh) + (lane >> sh);
@@ -295,6 +299,10 @@ __global__ __launch_bounds__(64) void k_hybrid(HybridArgs ha) {
     f[0] = fminf(fin[0], 1.0f);
     f[1] = fminf(fin[1], 1.0f);
     f[2] = fminf(fin[2], 1.0f);
+    if (a.cost && lane == 0) {
+        const uint64_t dt = (uint64_t)(wall_clock64() - (long long)t_start);
+        a.cost[L] = dt < 0xFFFFFFFFull ? (uint32_t)dt : 0xFFFFFFFFu;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -536,11 +544,12 @@ struct kdgrt_scene {
     // ordered by cone_ev like the cone table): d_sched holds sched_cap
     // durations, sched_cap smoothed durations and sched_cap tile indices (the
     // order for the next launch of the view in sched_cam / sched_w x sched_h /
-    // sched_depth).
+    // sched_depth, launched by the kernel of sched_kind).
     uint32_t* d_sched = nullptr;
     uint32_t sched_cap = 0;
     bool sched_on = true;
     bool sched_valid = false;
+    int sched_kind = 0; // 0 plain view kernel, 1 hybrid kernel
     int sched_w = 0, sched_h = 0, sched_depth = 0;
     CamInfo sched_cam{};
     uint64_t bytes = 0;
@@ -1114,8 +1123,8 @@ static kdgrt_status with_cone_table(const kdgrt_scene* csc, ViewArgs& a, hipStre
 // distance to the scene box centre, pixel footprint within 1 %.  (Orbiting
 // the 1M-Gaussian scene by 0.25 degrees per frame keeps about half of the
 // static view's gain; at 1 degree per frame none is left.)
-static bool sched_similar(const kdgrt_scene* sc, const ViewArgs& a) {
-    if (!sc->sched_valid || sc->sched_w != a.W || sc->sched_h != a.H || sc->sched_depth != a.depth) return false;
+static bool sched_similar(const kdgrt_scene* sc, const ViewArgs& a, int kind) {
+    if (!sc->sched_valid || sc->sched_kind != kind || sc->sched_w != a.W || sc->sched_h != a.H || sc->sched_depth != a.depth) return false;
     const CamInfo& p = sc->sched_cam;
     const CamInfo& c = a.cam;
     auto centre_ray = [&](const CamInfo& k, double r[3]) {
@@ -1156,6 +1165,43 @@ static bool sched_similar(const kdgrt_scene* sc, const ViewArgs& a) {
     return std::fabs(fx1 - fx0) <= 0.01 * fx0 && std::fabs(fy1 - fy0) <= 0.01 * fy0;
 }
 
+// Schedule state for a launch of n logical tiles by the kernel of `kind`
+// (inside with_cone_table): sets a.cost, and a.order when the scene's last
+// view is close to this one.  `use` is false when the schedule is off or the
+// launch is too large for the one-workgroup sort.
+static kdgrt_status sched_before(kdgrt_scene* sc, ViewArgs& a, uint32_t n, int kind, bool& use, bool& similar) {
+    use = sc->sched_on && n <= kSchedMaxTiles;
+    similar = false;
+    if (!use) return KDGRT_OK;
+    if (n > sc->sched_cap) {
+        // earlier launches of this scene (ordered by cone_ev) may still read
+        // the old buffer
+        if (sc->cone_used) HIP_TRY(hipEventSynchronize(sc->cone_ev));
+        HIP_TRY(hipFree(sc->d_sched));
+        sc->d_sched = nullptr;
+        sc->sched_cap = 0;
+        sc->sched_valid = false;
+        HIP_TRY(hipMalloc((void**)&sc->d_sched, 3ull * n * sizeof(uint32_t)));
+        sc->sched_cap = n;
+    }
+    similar = sched_similar(sc, a, kind);
+    a.cost = sc->d_sched;
+    a.order = similar ? sc->d_sched + 2ull * sc->sched_cap : nullptr;
+    return KDGRT_OK;
+}
+
+// After the launch: the tile order for the next launch of this view.
+static void sched_after(kdgrt_scene* sc, const ViewArgs& a, uint32_t n, int kind, bool similar, hipStream_t s) {
+    hipLaunchKernelGGL(k_tile_sched, dim3(1), dim3(1024), 0, s, sc->d_sched, sc->d_sched + sc->sched_cap,
+                       sc->d_sched + 2ull * sc->sched_cap, n, similar ? 0 : 1);
+    sc->sched_valid = true;
+    sc->sched_kind = kind;
+    sc->sched_w = a.W;
+    sc->sched_h = a.H;
+    sc->sched_depth = a.depth;
+    sc->sched_cam = a.cam;
+}
+
 typedef void (*ViewKernel)(ViewArgs);
 // sched: record tile durations and follow the scene's tile schedule (the
 // plain kernel only; debug and profile launches keep the raster order).
@@ -1163,34 +1209,13 @@ static kdgrt_status launch_view(const kdgrt_scene* csc, ViewArgs& a, ViewKernel 
                                 bool sched = false) {
     return with_cone_table(csc, a, s, [&](kdgrt_scene* sc) -> kdgrt_status {
         const uint32_t n = interleave_grid<64>((uint64_t)a.blocks_x * a.blocks_y * 64);
-        const bool use = sched && sc->sched_on && n <= kSchedMaxTiles;
-        bool similar = false;
-        if (use) {
-            if (n > sc->sched_cap) {
-                // earlier launches of this scene (ordered by cone_ev) may
-                // still read the old buffer
-                if (sc->cone_used) HIP_TRY(hipEventSynchronize(sc->cone_ev));
-                HIP_TRY(hipFree(sc->d_sched));
-                sc->d_sched = nullptr;
-                sc->sched_cap = 0;
-                sc->sched_valid = false;
-                HIP_TRY(hipMalloc((void**)&sc->d_sched, 3ull * n * sizeof(uint32_t)));
-                sc->sched_cap = n;
-            }
-            similar = sched_similar(sc, a);
-            a.cost = sc->d_sched;
-            a.order = similar ? sc->d_sched + 2ull * sc->sched_cap : nullptr;
+        bool use = false, similar = false;
+        if (sched) {
+            const kdgrt_status st = sched_before(sc, a, n, 0, use, similar);
+            if (st != KDGRT_OK) return st;
         }
         hipLaunchKernelGGL(kern, dim3(n), dim3(64), stack_lds_bytes(a.depth, 64), s, a);
-        if (use) {
-            hipLaunchKernelGGL(k_tile_sched, dim3(1), dim3(1024), 0, s, sc->d_sched, sc->d_sched + sc->sched_cap,
-                               sc->d_sched + 2ull * sc->sched_cap, n, similar ? 0 : 1);
-            sc->sched_valid = true;
-            sc->sched_w = a.W;
-            sc->sched_h = a.H;
-            sc->sched_depth = a.depth;
-            sc->sched_cam = a.cam;
-        }
+        if (use) sched_after(sc, a, n, 0, similar, s);
         return KDGRT_OK;
     });
 }
@@ -1275,9 +1300,13 @@ kdgrt_status kdgrt_render_hybrid(kdgrt_context* ctx, const kdgrt_scene* sc, cons
     }
     fill_view_args(ha.v, sc, cam, W, H, depth, out);
     const hipStream_t s = (hipStream_t)stream;
-    return with_cone_table(sc, ha.v, s, [&](kdgrt_scene*) -> kdgrt_status {
-        const dim3 grid(interleave_grid<64>((uint64_t)ha.v.blocks_x * ha.v.blocks_y * 64)), block(64);
-        hipLaunchKernelGGL(k_hybrid<kViewK>, grid, block, stack_lds_bytes(depth, 64), s, ha);
+    return with_cone_table(sc, ha.v, s, [&](kdgrt_scene* msc) -> kdgrt_status {
+        const uint32_t n = interleave_grid<64>((uint64_t)ha.v.blocks_x * ha.v.blocks_y * 64);
+        bool use = false, similar = false;
+        const kdgrt_status st = sched_before(msc, ha.v, n, 1, use, similar);
+        if (st != KDGRT_OK) return st;
+        hipLaunchKernelGGL(k_hybrid<kViewK>, dim3(n), dim3(64), stack_lds_bytes(depth, 64), s, ha);
+        if (use) sched_after(msc, ha.v, n, 1, similar, s);
         return KDGRT_OK;
     });
 }

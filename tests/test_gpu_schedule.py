@@ -87,3 +87,36 @@ def test_schedule_toggle_and_repeat_full_size(dev):
     sc.set_tile_schedule(True)
     assert np.array_equal(_frame(dev, sc, ci, 1920, 1080), ref)
     sc.free()
+
+
+def test_schedule_hybrid_and_interleaved_kernels(dev):
+    """The hybrid kernel follows the same schedule: hybrid frames with the
+    schedule equal raster-order hybrid frames, also when plain and hybrid
+    launches of one view alternate (each kernel keeps its own durations)."""
+    hs = S.clustered(100_000, seed=1012)
+    on = dev.scene(hs.tree, hs.g)
+    off = dev.scene(hs.tree, hs.g)
+    off.set_tile_schedule(False)
+    sph = S.spheres_in_view(hs.aabb)
+    w, h = 640, 480
+
+    def hyb(sc, ci):
+        buf = dev.upload(np.full((h, w, 3), np.nan, np.float32))
+        dev.render_hybrid(sc, ci, w, h, spheres=sph, out=buf)
+        dev.sync()
+        img = buf.download(np.float32, (h, w, 3))
+        buf.free()
+        return img
+
+    for k, (yaw, kind) in enumerate([(0.0, "h"), (0.0, "h"), (0.0, "h"), (0.2, "h"), (0.2, "p"), (0.2, "h"),
+                                     (0.2, "p"), (0.2, "p"), (30.0, "h"), (30.0, "h")]):
+        ci = _orbit(hs.aabb, w, h, yaw)
+        print(f"[schedule hybrid] frame {k}: yaw {yaw}, kernel {kind}", flush=True)
+        if kind == "h":
+            a, b = hyb(on, ci), hyb(off, ci)
+        else:
+            a, b = _frame(dev, on, ci, w, h), _frame(dev, off, ci, w, h)
+        assert not np.isnan(a).any(), f"frame {k}: pixels left unwritten"
+        assert np.array_equal(a, b), f"frame {k}: scheduled and raster frames differ"
+    on.free()
+    off.free()
