@@ -146,17 +146,18 @@ __global__ __launch_bounds__(64) void k_view(ViewArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Tile order of the next launch of a view (frame-to-frame schedule).
+// Tile order of a launch that follows a close view (frame-to-frame schedule).
 // ---------------------------------------------------------------------------
 // A launch lasts at least as long as its slowest tile after that tile
 // starts, so tiles that start late and run long leave the chip idle at the
 // end (a 1080p view of the 1M-Gaussian scene keeps 77 % of its wave slots
-// busy in raster order).  k_view records the duration of every tile; this
-// single-workgroup counting sort turns the smoothed durations (mean of the
-// last value and this one, the plain value after a camera change) into a
-// longest-first order for the next launch of the same view.  Durations fall
-// into 512 log-spaced classes, 16 per octave; order within a class is that of
-// the LDS atomics.  Any duration values give a permutation of [0, n).
+// busy in raster order).  k_view records the duration of every tile; before
+// the next launch of a close view, this single-workgroup counting sort turns
+// the smoothed durations (mean of the previous smoothed value and the last
+// duration; the last duration alone after a camera change) into a
+// longest-first order.  Durations fall into 512 log-spaced classes, 16 per
+// octave; order within a class is that of the LDS atomics.  Any duration
+// values give a permutation of [0, n).
 constexpr int kSchedClasses = 512;
 constexpr uint32_t kSchedMaxTiles = 1u << 20;
 
@@ -173,11 +174,26 @@ __global__ __launch_bounds__(1024) void k_tile_sched(const uint32_t* __restrict_
     const uint32_t tid = threadIdx.x;
     for (uint32_t k = tid; k < (uint32_t)kSchedClasses; k += blockDim.x) cls[k] = 0;
     __syncthreads();
-    for (uint32_t t = tid; t < n; t += blockDim.x) {
-        const uint32_t c = cost[t], e = smooth[t];
-        const uint32_t v = (reset || e == 0) ? c : (uint32_t)(((uint64_t)e + c + 1u) >> 1);
-        smooth[t] = v;
-        atomicAdd(&cls[sched_class(v)], 1u);
+    // eight elements per thread and step, loaded before any is used, so a
+    // step waits for one memory round trip rather than eight
+    constexpr uint32_t kU = 8;
+    const uint32_t stride = kU * blockDim.x;
+    for (uint32_t b0 = 0; b0 < n; b0 += stride) {
+        uint32_t c[kU], e[kU];
+#pragma unroll
+        for (uint32_t j = 0; j < kU; ++j) {
+            const uint32_t t = b0 + j * blockDim.x + tid;
+            c[j] = t < n ? cost[t] : 0u;
+            e[j] = t < n ? smooth[t] : 0u;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kU; ++j) {
+            const uint32_t t = b0 + j * blockDim.x + tid;
+            if (t >= n) break;
+            const uint32_t v = (reset || e[j] == 0) ? c[j] : (uint32_t)(((uint64_t)e[j] + c[j] + 1u) >> 1);
+            smooth[t] = v;
+            atomicAdd(&cls[sched_class(v)], 1u);
+        }
     }
     __syncthreads();
     if (tid < 64) {
@@ -203,7 +219,20 @@ __global__ __launch_bounds__(1024) void k_tile_sched(const uint32_t* __restrict_
         }
     }
     __syncthreads();
-    for (uint32_t t = tid; t < n; t += blockDim.x) order[atomicAdd(&cls[sched_class(smooth[t])], 1u)] = t;
+    for (uint32_t b0 = 0; b0 < n; b0 += stride) {
+        uint32_t v[kU];
+#pragma unroll
+        for (uint32_t j = 0; j < kU; ++j) {
+            const uint32_t t = b0 + j * blockDim.x + tid;
+            v[j] = t < n ? smooth[t] : 0u;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kU; ++j) {
+            const uint32_t t = b0 + j * blockDim.x + tid;
+            if (t >= n) break;
+            order[atomicAdd(&cls[sched_class(v[j])], 1u)] = t;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -540,15 +569,17 @@ struct kdgrt_scene {
     hipStream_t cone_stream = nullptr;
     bool cone_used = false;
     std::mutex cone_mu;
-    // Frame-to-frame tile schedule of kdgrt_render (guarded by cone_mu and
-    // ordered by cone_ev like the cone table): d_sched holds sched_cap
-    // durations, sched_cap smoothed durations and sched_cap tile indices (the
-    // order for the next launch of the view in sched_cam / sched_w x sched_h /
-    // sched_depth, launched by the kernel of sched_kind).
+    // Frame-to-frame tile schedule of kdgrt_render and kdgrt_render_hybrid
+    // (guarded by cone_mu and ordered by cone_ev like the cone table):
+    // d_sched holds sched_cap tile durations of the last launch (the view in
+    // sched_cam / sched_w x sched_h / sched_depth, by the kernel of
+    // sched_kind), sched_cap smoothed durations and sched_cap tile indices
+    // (the order of a launch that follows a close view).
     uint32_t* d_sched = nullptr;
     uint32_t sched_cap = 0;
     bool sched_on = true;
     bool sched_valid = false;
+    bool sched_fresh = true; // the smoothed durations are to be restarted from the next costs
     int sched_kind = 0; // 0 plain view kernel, 1 hybrid kernel
     int sched_w = 0, sched_h = 0, sched_depth = 0;
     CamInfo sched_cam{};
@@ -1166,15 +1197,17 @@ static bool sched_similar(const kdgrt_scene* sc, const ViewArgs& a, int kind) {
 }
 
 // Schedule state for a launch of n logical tiles by the kernel of `kind`
-// (inside with_cone_table): sets a.cost, and a.order when the scene's last
-// view is close to this one.  `use` is false when the schedule is off or the
+// (inside with_cone_table, on stream s).  When the scene's last launch was
+// of a view close to this one, k_tile_sched first turns that launch's tile
+// durations into this launch's order (a.order); otherwise the launch runs in
+// raster order and no sort is paid for.  Either way the launch records its
+// own durations (a.cost).  `use` is false when the schedule is off or the
 // launch is too large for the one-workgroup sort.
-static kdgrt_status sched_before(kdgrt_scene* sc, ViewArgs& a, uint32_t n, int kind, bool& use, bool& similar) {
+static kdgrt_status sched_before(kdgrt_scene* sc, ViewArgs& a, uint32_t n, int kind, hipStream_t s, bool& use) {
     use = sc->sched_on && n <= kSchedMaxTiles;
-    similar = false;
     if (!use) return KDGRT_OK;
     if (n > sc->sched_cap) {
-        // earlier launches of this scene (ordered by cone_ev) may still read
+        // earlier launches of this scene (ordered by cone_ev) may still use
         // the old buffer
         if (sc->cone_used) HIP_TRY(hipEventSynchronize(sc->cone_ev));
         HIP_TRY(hipFree(sc->d_sched));
@@ -1184,16 +1217,23 @@ static kdgrt_status sched_before(kdgrt_scene* sc, ViewArgs& a, uint32_t n, int k
         HIP_TRY(hipMalloc((void**)&sc->d_sched, 3ull * n * sizeof(uint32_t)));
         sc->sched_cap = n;
     }
-    similar = sched_similar(sc, a, kind);
-    a.cost = sc->d_sched;
-    a.order = similar ? sc->d_sched + 2ull * sc->sched_cap : nullptr;
+    uint32_t* cost = sc->d_sched;
+    uint32_t* smooth = sc->d_sched + sc->sched_cap;
+    uint32_t* order = sc->d_sched + 2ull * sc->sched_cap;
+    a.order = nullptr;
+    if (sched_similar(sc, a, kind)) {
+        hipLaunchKernelGGL(k_tile_sched, dim3(1), dim3(1024), 0, s, cost, smooth, order, n, sc->sched_fresh ? 1 : 0);
+        sc->sched_fresh = false;
+        a.order = order;
+    } else {
+        sc->sched_fresh = true; // the smoothed durations belong to another view
+    }
+    a.cost = cost;
     return KDGRT_OK;
 }
 
-// After the launch: the tile order for the next launch of this view.
-static void sched_after(kdgrt_scene* sc, const ViewArgs& a, uint32_t n, int kind, bool similar, hipStream_t s) {
-    hipLaunchKernelGGL(k_tile_sched, dim3(1), dim3(1024), 0, s, sc->d_sched, sc->d_sched + sc->sched_cap,
-                       sc->d_sched + 2ull * sc->sched_cap, n, similar ? 0 : 1);
+// After the launch: the view whose durations the cost buffer now holds.
+static void sched_after(kdgrt_scene* sc, const ViewArgs& a, int kind) {
     sc->sched_valid = true;
     sc->sched_kind = kind;
     sc->sched_w = a.W;
@@ -1209,13 +1249,13 @@ static kdgrt_status launch_view(const kdgrt_scene* csc, ViewArgs& a, ViewKernel 
                                 bool sched = false) {
     return with_cone_table(csc, a, s, [&](kdgrt_scene* sc) -> kdgrt_status {
         const uint32_t n = interleave_grid<64>((uint64_t)a.blocks_x * a.blocks_y * 64);
-        bool use = false, similar = false;
+        bool use = false;
         if (sched) {
-            const kdgrt_status st = sched_before(sc, a, n, 0, use, similar);
+            const kdgrt_status st = sched_before(sc, a, n, 0, s, use);
             if (st != KDGRT_OK) return st;
         }
         hipLaunchKernelGGL(kern, dim3(n), dim3(64), stack_lds_bytes(a.depth, 64), s, a);
-        if (use) sched_after(sc, a, n, 0, similar, s);
+        if (use) sched_after(sc, a, 0);
         return KDGRT_OK;
     });
 }
@@ -1302,11 +1342,11 @@ kdgrt_status kdgrt_render_hybrid(kdgrt_context* ctx, const kdgrt_scene* sc, cons
     const hipStream_t s = (hipStream_t)stream;
     return with_cone_table(sc, ha.v, s, [&](kdgrt_scene* msc) -> kdgrt_status {
         const uint32_t n = interleave_grid<64>((uint64_t)ha.v.blocks_x * ha.v.blocks_y * 64);
-        bool use = false, similar = false;
-        const kdgrt_status st = sched_before(msc, ha.v, n, 1, use, similar);
+        bool use = false;
+        const kdgrt_status st = sched_before(msc, ha.v, n, 1, s, use);
         if (st != KDGRT_OK) return st;
         hipLaunchKernelGGL(k_hybrid<kViewK>, dim3(n), dim3(64), stack_lds_bytes(depth, 64), s, ha);
-        if (use) sched_after(msc, ha.v, n, 1, similar, s);
+        if (use) sched_after(msc, ha.v, 1);
         return KDGRT_OK;
     });
 }
