@@ -120,3 +120,29 @@ def test_schedule_hybrid_and_interleaved_kernels(dev):
         assert np.array_equal(a, b), f"frame {k}: scheduled and raster frames differ"
     on.free()
     off.free()
+
+
+def test_schedule_across_streams(dev):
+    """Launches of one scene alternate between two streams without host
+    waits in between: each launch's tile order comes from the durations the
+    other stream's launch recorded, ordered by the scene's event.  Every
+    frame equals the raster-order frame."""
+    hs = S.clustered(100_000, seed=1013)
+    sc = dev.scene(hs.tree, hs.g)
+    w, h = 640, 480
+    ci = _orbit(hs.aabb, w, h, 0.0)
+    sc.set_tile_schedule(False)
+    ref = _frame(dev, sc, ci, w, h)
+    sc.set_tile_schedule(True)
+    streams = [dev.stream(), dev.stream()]
+    bufs = [dev.upload(np.full((h, w, 4), np.nan, np.float32)) for _ in range(8)]
+    for k, b in enumerate(bufs):
+        dev.render(sc, ci, w, h, b, stream=streams[k % 2])
+    dev.sync()
+    for k, b in enumerate(bufs):
+        img = b.download(np.float32, (h, w, 4))
+        assert np.array_equal(img, ref), f"launch {k} (stream {k % 2}) differs"
+        b.free()
+    for st in streams:
+        K.check(K.lib().kdgrt_stream_destroy(dev.h, st))
+    sc.free()
