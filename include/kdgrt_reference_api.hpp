@@ -19,7 +19,10 @@
 //
 // Differences: device errors are reported on stderr and by the return value
 // (false / negative FPS) instead of exit(); the scene AABB is taken from the
-// CompositeObject as the reference does; the stream is a hipStream_t.
+// CompositeObject as the reference does; the stream is a hipStream_t.  Frames
+// of an unchanged or slowly moving camera start the tiles that ran longest in
+// the previous frame first (kdgrt_scene_set_tile_schedule); the image is the
+// same as in raster order.
 #pragma once
 
 #include <hip/hip_runtime.h>
