@@ -1263,8 +1263,8 @@ static kdgrt_status launch_view(const kdgrt_scene* csc, ViewArgs& a, ViewKernel 
     });
 }
 
-kdgrt_status kdgrt_render(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgrt_camera_info* cam, int W, int H,
-                          int depth, int fmt, void* out, void* stream) {
+static kdgrt_status render_plain(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgrt_camera_info* cam, int W,
+                                 int H, int depth, int fmt, void* out, void* stream, bool sched) {
     kdgrt_status st = check_view(ctx, sc, cam, W, H, depth, out);
     if (st != KDGRT_OK) return st;
     if (fmt != KDGRT_OUT_RGBT_F32 && fmt != KDGRT_OUT_RGB_F32)
@@ -1274,8 +1274,22 @@ kdgrt_status kdgrt_render(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgrt
     return launch_view(sc, a,
                        fmt == KDGRT_OUT_RGBT_F32 ? k_view<kViewK, false, KDGRT_OUT_RGBT_F32>
                                                  : k_view<kViewK, false, KDGRT_OUT_RGB_F32>,
-                       (hipStream_t)stream, true);
+                       (hipStream_t)stream, sched);
 }
+
+kdgrt_status kdgrt_render(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgrt_camera_info* cam, int W, int H,
+                          int depth, int fmt, void* out, void* stream) {
+    return render_plain(ctx, sc, cam, W, H, depth, fmt, out, stream, true);
+}
+
+extern "C++" {
+namespace kdgrt {
+kdgrt_status render_unscheduled(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgrt_camera_info* cam, int W, int H,
+                                int depth, void* out, hipStream_t stream) {
+    return render_plain(ctx, sc, cam, W, H, depth, KDGRT_OUT_RGBT_F32, out, stream, false);
+}
+} // namespace kdgrt
+} // extern "C++"
 
 kdgrt_status kdgrt_render_debug(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgrt_camera_info* cam, int W,
                                 int H, int depth, float* out, int32_t* counters, uint64_t* hashes, uint32_t* seq,
@@ -1399,7 +1413,8 @@ kdgrt_status kdgrt_warm_up(kdgrt_context* ctx, const kdgrt_scene* sc, void* stre
     cam.near_c = 0.005;
     kdgrt_camera_info ci;
     kdgrt_camera_make(&cam, 64, 64, &ci);
-    kdgrt_status st = kdgrt_render(ctx, sc, &ci, 64, 64, 12, KDGRT_OUT_RGBT_F32, tmp, stream);
+    // raster order: the warm-up view leaves the scene's tile schedule alone
+    kdgrt_status st = render_plain(ctx, sc, &ci, 64, 64, 12, KDGRT_OUT_RGBT_F32, tmp, stream, false);
     hipStreamSynchronize((hipStream_t)stream);
     hipFree(tmp);
     return st;

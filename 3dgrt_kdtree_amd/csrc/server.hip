@@ -266,9 +266,11 @@ bool launch_one(kdgrt_server* s) {
             // comparison mode, or a batch of one view: the one-view kernel
             // (its per-view cone pre-test makes it the faster path for a
             // single view); it still runs on the slot's stream and overlaps
-            // with the other slots
-            stt = kdgrt_render(s->ctx, views[0].scene, &views[0].cam, views[0].width, views[0].height,
-                               s->cfg.tuning.stack_depth, KDGRT_OUT_RGBT_F32, views[0].out_rgbt, st);
+            // with the other slots.  Raster tile order: the frame-to-frame
+            // schedule allocates its state on first use of a size, which the
+            // launch path must not do.
+            stt = kdgrt::render_unscheduled(s->ctx, views[0].scene, &views[0].cam, views[0].width, views[0].height,
+                                            s->cfg.tuning.stack_depth, views[0].out_rgbt, st);
         } else {
             uint64_t rays = 0;
             stt = kdgrt::slot_launch(s->slots[slot], views.data(), (int)views.size(), s->cfg.tuning, st, &rays);
