@@ -1158,7 +1158,9 @@ static kdgrt_status with_cone_table(const kdgrt_scene* csc, ViewArgs& a, hipStre
 // the 1M-Gaussian scene by 0.25 degrees per frame keeps about half of the
 // static view's gain; at 1 degree per frame none is left.)
 static bool sched_similar(const kdgrt_scene* sc, const ViewArgs& a, int kind) {
-    if (!sc->sched_valid || sc->sched_kind != kind || sc->sched_w != a.W || sc->sched_h != a.H || sc->sched_depth != a.depth) return false;
+    if (!sc->sched_valid || sc->sched_kind != kind || sc->sched_w != a.W || sc->sched_h != a.H ||
+        sc->sched_depth != a.depth)
+        return false;
     const CamInfo& p = sc->sched_cam;
     const CamInfo& c = a.cam;
     auto centre_ray = [&](const CamInfo& k, double r[3]) {
