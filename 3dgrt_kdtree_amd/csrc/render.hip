@@ -152,13 +152,14 @@ __global__ __launch_bounds__(64) void k_view(ViewArgs a) {
 // starts, so tiles that start late and run long leave the chip idle at the
 // end (a 1080p view of the 1M-Gaussian scene keeps 77 % of its wave slots
 // busy in raster order).  k_view records the duration of every tile; before
-// the next launch of a close view, this single-workgroup counting sort turns
-// the smoothed durations (mean of the previous smoothed value and the last
-// duration; the last duration alone after a camera change) into a
-// longest-first order.  Durations fall into 512 log-spaced classes, 16 per
-// octave; order within a class is that of the LDS atomics.  Any duration
-// values give a permutation of [0, n).
+// the next launch of a close view, a counting sort run by one workgroup (the
+// last workgroup of k_cone_prep) turns the smoothed durations (mean of the
+// previous smoothed value and the last duration; the last duration alone
+// after a camera change) into a longest-first order.  Durations fall into
+// 512 log-spaced classes, 16 per octave; order within a class is that of
+// the LDS atomics.  Any duration values give a permutation of [0, n).
 constexpr int kSchedClasses = 512;
+constexpr uint32_t kCinBlock = 1024; // threads per workgroup of k_cone_prep
 constexpr uint32_t kSchedMaxTiles = 1u << 20;
 
 __device__ __forceinline__ uint32_t sched_class(uint32_t c) {
@@ -168,9 +169,23 @@ __device__ __forceinline__ uint32_t sched_class(uint32_t c) {
     return e * 16u + m;
 }
 
-__global__ __launch_bounds__(1024) void k_tile_sched(const uint32_t* __restrict__ cost, uint32_t* __restrict__ smooth,
-                                                     uint32_t* __restrict__ order, uint32_t n, int reset) {
+struct TileSort {
+    const uint32_t* cost; // durations of the previous launch, per logical tile
+    uint32_t* smooth;     // smoothed durations (in/out)
+    uint32_t* order;      // out: tile order of the coming launch
+    uint32_t n;           // logical tiles
+    int reset;            // smooth := cost (the previous view was another one)
+    int on;               // run the sort at all
+};
+
+// The sort, run by one workgroup of kCinBlock threads.
+__device__ void tile_sort_block(const TileSort& ts) {
     __shared__ uint32_t cls[kSchedClasses];
+    const uint32_t* __restrict__ cost = ts.cost;
+    uint32_t* __restrict__ smooth = ts.smooth;
+    uint32_t* __restrict__ order = ts.order;
+    const uint32_t n = ts.n;
+    const int reset = ts.reset;
     const uint32_t tid = threadIdx.x;
     for (uint32_t k = tid; k < (uint32_t)kSchedClasses; k += blockDim.x) cls[k] = 0;
     __syncthreads();
@@ -419,9 +434,19 @@ __global__ __launch_bounds__(256) void k_copy(float4* __restrict__ dst, const fl
 // a.d >= Q and the f32 a.d >= q = Q - 4eps|a|.  q is formed in double and
 // rounded down; otherwise (origin near or inside the sphere, or r = inf) the
 // record accepts every direction.
-__global__ __launch_bounds__(256) void k_cone_prep(const float4* __restrict__ lsph, uint64_t n, float ex, float ey,
-                                                   float ez, float4* __restrict__ vcone) {
-    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+// With ts.on, the first workgroup (dispatched first, so it starts at once)
+// sorts the tile order of the coming view launch instead (tile_sort_block)
+// while the others stream the cone records, so the sort adds no time of its
+// own.
+__global__ __launch_bounds__(kCinBlock) void k_cone_prep(const float4* __restrict__ lsph, uint64_t n, float ex, float ey,
+                                                         float ez, float4* __restrict__ vcone, TileSort ts) {
+    const uint32_t first = ts.on ? 1u : 0u, prep_blocks = gridDim.x - first;
+    if (blockIdx.x < first) {
+        tile_sort_block(ts);
+        return;
+    }
+    for (uint64_t i = (uint64_t)(blockIdx.x - first) * kCinBlock + threadIdx.x; i < n;
+         i += (uint64_t)prep_blocks * kCinBlock) {
         const float4 sp = lsph[i];
         const float ax = sp.x - ex, ay = sp.y - ey, az = sp.z - ez;
         const uint32_t w = __float_as_uint(sp.w);
@@ -1126,30 +1151,10 @@ static void fill_view_args(ViewArgs& a, const kdgrt_scene* sc, const kdgrt_camer
     a.seq_cap = 0;
 }
 
-// Launches one view: the cone records of the view's origin into the scene's
-// table, then `launch` (the view kernel, reading a.sc.vcone), both on `s`.
-// Launches of one scene on different streams are ordered through the
-// table's event.
-extern "C++" {
-template <class Launch>
-static kdgrt_status with_cone_table(const kdgrt_scene* csc, ViewArgs& a, hipStream_t s, Launch&& launch) {
-    kdgrt_scene* sc = const_cast<kdgrt_scene*>(csc);
-    std::lock_guard<std::mutex> lock(sc->cone_mu);
-    if (sc->cone_used && sc->cone_stream != s) HIP_TRY(hipStreamWaitEvent(s, sc->cone_ev, 0));
-    const uint64_t n = sc->n_offsets + kLsphPad;
-    const uint32_t prep_grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_cone_prep, dim3(prep_grid), dim3(256), 0, s, (const float4*)sc->d_lsph, n, a.cam.eye[0],
-                       a.cam.eye[1], a.cam.eye[2], (float4*)sc->d_vcone);
-    a.sc.vcone = (const float4*)sc->d_vcone;
-    const kdgrt_status st = launch(sc);
-    if (st != KDGRT_OK) return st;
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(sc->cone_ev, s));
-    sc->cone_stream = s;
-    sc->cone_used = true;
-    return KDGRT_OK;
+// Logical tiles of a view launch (the grid of k_view and k_hybrid).
+static uint32_t view_tiles(const ViewArgs& a) {
+    return interleave_grid<64>((uint64_t)a.blocks_x * a.blocks_y * 64);
 }
-} // extern "C++"
 
 // Whether the previous view of the scene is close enough to this one for its
 // tile durations to order this launch: same size and stack depth, image axes
@@ -1202,14 +1207,15 @@ static bool sched_similar(const kdgrt_scene* sc, const ViewArgs& a, int kind) {
 }
 
 // Schedule state for a launch of n logical tiles by the kernel of `kind`
-// (inside with_cone_table, on stream s).  When the scene's last launch was
-// of a view close to this one, k_tile_sched first turns that launch's tile
+// (inside with_cone_table).  When the scene's last launch was of a view
+// close to this one, `ts` asks k_cone_prep to turn that launch's tile
 // durations into this launch's order (a.order); otherwise the launch runs in
-// raster order and no sort is paid for.  Either way the launch records its
-// own durations (a.cost).  `use` is false when the schedule is off or the
-// launch is too large for the one-workgroup sort.
-static kdgrt_status sched_before(kdgrt_scene* sc, ViewArgs& a, uint32_t n, int kind, hipStream_t s, bool& use) {
+// raster order and no sort is run.  Either way the launch records its own
+// durations (a.cost).  `use` is false when the schedule is off or the launch
+// is too large for the one-workgroup sort.
+static kdgrt_status sched_before(kdgrt_scene* sc, ViewArgs& a, uint32_t n, int kind, bool& use, TileSort& ts) {
     use = sc->sched_on && n <= kSchedMaxTiles;
+    ts.on = 0;
     if (!use) return KDGRT_OK;
     if (n > sc->sched_cap) {
         // earlier launches of this scene (ordered by cone_ev) may still use
@@ -1227,7 +1233,7 @@ static kdgrt_status sched_before(kdgrt_scene* sc, ViewArgs& a, uint32_t n, int k
     uint32_t* order = sc->d_sched + 2ull * sc->sched_cap;
     a.order = nullptr;
     if (sched_similar(sc, a, kind)) {
-        hipLaunchKernelGGL(k_tile_sched, dim3(1), dim3(1024), 0, s, cost, smooth, order, n, sc->sched_fresh ? 1 : 0);
+        ts = TileSort{cost, smooth, order, n, sc->sched_fresh ? 1 : 0, 1};
         sc->sched_fresh = false;
         a.order = order;
     } else {
@@ -1247,20 +1253,49 @@ static void sched_after(kdgrt_scene* sc, const ViewArgs& a, int kind) {
     sc->sched_cam = a.cam;
 }
 
+// Launches one view: the cone records of the view's origin into the scene's
+// table, then `launch` (the view kernel, reading a.sc.vcone), both on `s`.
+// Launches of one scene on different streams are ordered through the
+// table's event.
+extern "C++" {
+// sched_kind >= 0: the launch records tile durations and follows the
+// scene's tile schedule for that kernel (0 plain, 1 hybrid); the tile sort,
+// when one is due, runs inside k_cone_prep.
+template <class Launch>
+static kdgrt_status with_cone_table(const kdgrt_scene* csc, ViewArgs& a, hipStream_t s, int sched_kind,
+                                    Launch&& launch) {
+    kdgrt_scene* sc = const_cast<kdgrt_scene*>(csc);
+    std::lock_guard<std::mutex> lock(sc->cone_mu);
+    if (sc->cone_used && sc->cone_stream != s) HIP_TRY(hipStreamWaitEvent(s, sc->cone_ev, 0));
+    TileSort ts{};
+    bool use = false;
+    if (sched_kind >= 0) {
+        const kdgrt_status st = sched_before(sc, a, view_tiles(a), sched_kind, use, ts);
+        if (st != KDGRT_OK) return st;
+    }
+    const uint64_t n = sc->n_offsets + kLsphPad;
+    const uint32_t prep_grid = (uint32_t)std::min<uint64_t>((n + kCinBlock - 1) / kCinBlock, 2048) + (ts.on ? 1 : 0);
+    hipLaunchKernelGGL(k_cone_prep, dim3(prep_grid), dim3(kCinBlock), 0, s, (const float4*)sc->d_lsph, n,
+                       a.cam.eye[0], a.cam.eye[1], a.cam.eye[2], (float4*)sc->d_vcone, ts);
+    a.sc.vcone = (const float4*)sc->d_vcone;
+    const kdgrt_status st = launch(sc);
+    if (st != KDGRT_OK) return st;
+    HIP_TRY(hipGetLastError());
+    if (use) sched_after(sc, a, sched_kind);
+    HIP_TRY(hipEventRecord(sc->cone_ev, s));
+    sc->cone_stream = s;
+    sc->cone_used = true;
+    return KDGRT_OK;
+}
+} // extern "C++"
+
 typedef void (*ViewKernel)(ViewArgs);
 // sched: record tile durations and follow the scene's tile schedule (the
 // plain kernel only; debug and profile launches keep the raster order).
 static kdgrt_status launch_view(const kdgrt_scene* csc, ViewArgs& a, ViewKernel kern, hipStream_t s,
                                 bool sched = false) {
-    return with_cone_table(csc, a, s, [&](kdgrt_scene* sc) -> kdgrt_status {
-        const uint32_t n = interleave_grid<64>((uint64_t)a.blocks_x * a.blocks_y * 64);
-        bool use = false;
-        if (sched) {
-            const kdgrt_status st = sched_before(sc, a, n, 0, s, use);
-            if (st != KDGRT_OK) return st;
-        }
-        hipLaunchKernelGGL(kern, dim3(n), dim3(64), stack_lds_bytes(a.depth, 64), s, a);
-        if (use) sched_after(sc, a, 0);
+    return with_cone_table(csc, a, s, sched ? 0 : -1, [&](kdgrt_scene*) -> kdgrt_status {
+        hipLaunchKernelGGL(kern, dim3(view_tiles(a)), dim3(64), stack_lds_bytes(a.depth, 64), s, a);
         return KDGRT_OK;
     });
 }
@@ -1359,13 +1394,8 @@ kdgrt_status kdgrt_render_hybrid(kdgrt_context* ctx, const kdgrt_scene* sc, cons
     }
     fill_view_args(ha.v, sc, cam, W, H, depth, out);
     const hipStream_t s = (hipStream_t)stream;
-    return with_cone_table(sc, ha.v, s, [&](kdgrt_scene* msc) -> kdgrt_status {
-        const uint32_t n = interleave_grid<64>((uint64_t)ha.v.blocks_x * ha.v.blocks_y * 64);
-        bool use = false;
-        const kdgrt_status st = sched_before(msc, ha.v, n, 1, s, use);
-        if (st != KDGRT_OK) return st;
-        hipLaunchKernelGGL(k_hybrid<kViewK>, dim3(n), dim3(64), stack_lds_bytes(depth, 64), s, ha);
-        if (use) sched_after(msc, ha.v, 1);
+    return with_cone_table(sc, ha.v, s, 1, [&](kdgrt_scene*) -> kdgrt_status {
+        hipLaunchKernelGGL(k_hybrid<kViewK>, dim3(view_tiles(ha.v)), dim3(64), stack_lds_bytes(depth, 64), s, ha);
         return KDGRT_OK;
     });
 }

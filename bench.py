@@ -438,7 +438,7 @@ def main():
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": load_traffic("plain_1080p_1M"),
             "algorithmic_bytes_per_launch": B, "bytes_per_ray": round(B / rays, 1),
-            "kernel": "k_view<8, false, 0, false, true> (timed with its k_cone_prep and k_tile_sched)"}
+            "kernel": "k_view<8, false, 0, false, true> (timed with its k_cone_prep, which also sorts the tile order)"}
 
     phases = None
     if args.phases:
