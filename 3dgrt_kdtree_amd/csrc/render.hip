@@ -159,7 +159,12 @@ __global__ __launch_bounds__(64) void k_view(ViewArgs a) {
 // 512 log-spaced classes, 16 per octave; order within a class is that of
 // the LDS atomics.  Any duration values give a permutation of [0, n).
 constexpr int kSchedClasses = 512;
-constexpr uint32_t kCinBlock = 1024; // threads per workgroup of k_cone_prep
+// k_cone_prep<false> runs in workgroups of 256 threads with no LDS and few
+// registers, so it fits into the gaps that concurrent launches leave on the
+// CUs; k_cone_prep<true> runs in workgroups of 1024, the first of which sorts
+// a tile order (the sort is written for 1024 threads).
+constexpr uint32_t kCinBlock = 1024;
+constexpr uint32_t kCinBlockSmall = 256;
 constexpr uint32_t kSchedMaxTiles = 1u << 20;
 
 __device__ __forceinline__ uint32_t sched_class(uint32_t c) {
@@ -434,19 +439,21 @@ __global__ __launch_bounds__(256) void k_copy(float4* __restrict__ dst, const fl
 // a.d >= Q and the f32 a.d >= q = Q - 4eps|a|.  q is formed in double and
 // rounded down; otherwise (origin near or inside the sphere, or r = inf) the
 // record accepts every direction.
-// With ts.on, the first workgroup (dispatched first, so it starts at once)
-// sorts the tile order of the coming view launch instead (tile_sort_block)
-// while the others stream the cone records, so the sort adds no time of its
-// own.
-__global__ __launch_bounds__(kCinBlock) void k_cone_prep(const float4* __restrict__ lsph, uint64_t n, float ex, float ey,
-                                                         float ez, float4* __restrict__ vcone, TileSort ts) {
-    const uint32_t first = ts.on ? 1u : 0u, prep_blocks = gridDim.x - first;
-    if (blockIdx.x < first) {
+// SORT: the first workgroup (dispatched first, so it starts at once) sorts
+// the tile order of the coming view launch instead (tile_sort_block) while
+// the others stream the cone records, so the sort adds no time of its own.
+template <bool SORT>
+__global__ __launch_bounds__(SORT ? kCinBlock : kCinBlockSmall) void k_cone_prep(const float4* __restrict__ lsph,
+                                                                                 uint64_t n, float ex, float ey,
+                                                                                 float ez, float4* __restrict__ vcone,
+                                                                                 TileSort ts) {
+    const uint32_t first = SORT ? 1u : 0u, prep_blocks = gridDim.x - first;
+    if (SORT && blockIdx.x < first) {
         tile_sort_block(ts);
         return;
     }
-    for (uint64_t i = (uint64_t)(blockIdx.x - first) * kCinBlock + threadIdx.x; i < n;
-         i += (uint64_t)prep_blocks * kCinBlock) {
+    for (uint64_t i = (uint64_t)(blockIdx.x - first) * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)prep_blocks * blockDim.x) {
         const float4 sp = lsph[i];
         const float ax = sp.x - ex, ay = sp.y - ey, az = sp.z - ez;
         const uint32_t w = __float_as_uint(sp.w);
@@ -1274,9 +1281,15 @@ static kdgrt_status with_cone_table(const kdgrt_scene* csc, ViewArgs& a, hipStre
         if (st != KDGRT_OK) return st;
     }
     const uint64_t n = sc->n_offsets + kLsphPad;
-    const uint32_t prep_grid = (uint32_t)std::min<uint64_t>((n + kCinBlock - 1) / kCinBlock, 2048) + (ts.on ? 1 : 0);
-    hipLaunchKernelGGL(k_cone_prep, dim3(prep_grid), dim3(kCinBlock), 0, s, (const float4*)sc->d_lsph, n,
-                       a.cam.eye[0], a.cam.eye[1], a.cam.eye[2], (float4*)sc->d_vcone, ts);
+    const uint32_t blk = ts.on ? kCinBlock : kCinBlockSmall;
+    const uint32_t prep_grid =
+        (uint32_t)std::min<uint64_t>((n + blk - 1) / blk, (2048u * kCinBlock) / blk) + (ts.on ? 1u : 0u);
+    if (ts.on)
+        hipLaunchKernelGGL(k_cone_prep<true>, dim3(prep_grid), dim3(blk), 0, s, (const float4*)sc->d_lsph, n,
+                           a.cam.eye[0], a.cam.eye[1], a.cam.eye[2], (float4*)sc->d_vcone, ts);
+    else
+        hipLaunchKernelGGL(k_cone_prep<false>, dim3(prep_grid), dim3(blk), 0, s, (const float4*)sc->d_lsph, n,
+                           a.cam.eye[0], a.cam.eye[1], a.cam.eye[2], (float4*)sc->d_vcone, ts);
     a.sc.vcone = (const float4*)sc->d_vcone;
     const kdgrt_status st = launch(sc);
     if (st != KDGRT_OK) return st;
