@@ -61,6 +61,8 @@ def test_errors_are_codes_not_exits():
     h = C.c_void_p()
     assert L.kdgrt_tree_build(None, C.c_uint64(0), None, C.byref(h)) == 1
     assert L.kdgrt_kdt_read(b"/nonexistent/x.kdt", C.byref(h)) == 4
+    assert L.kdgrt_scene_set_tile_schedule(None, C.c_int(1)) == 1
+    assert b"null scene" in L.kdgrt_last_error()
 
 
 def test_init_without_device_reports_no_device():
