@@ -174,18 +174,44 @@ __device__ __forceinline__ uint32_t sched_class(uint32_t c) {
     return e * 16u + m;
 }
 
+// Density cells of the first-launch estimate: cells of 2^cell_shift x
+// 2^cell_shift tiles, at most kDensCells of them (32 KB of LDS in the sort).
+constexpr uint32_t kDensCells = 8192;
+
 struct TileSort {
     const uint32_t* cost; // durations of the previous launch, per logical tile
     uint32_t* smooth;     // smoothed durations (in/out)
     uint32_t* order;      // out: tile order of the coming launch
+    uint32_t* dens;       // density mode: Gaussian centres per cell (zeroed after reading)
     uint32_t n;           // logical tiles
     int reset;            // smooth := cost (the previous view was another one)
     int on;               // run the sort at all
+    int density;          // key by the centre density around each tile instead of durations
+    int blocks_x;         // 64-tile groups per image row (logical tile -> tile position)
+    int tiles_w, tiles_h; // tiles of the image
+    int cell_shift, cells_w, cells_h;
 };
 
-// The sort, run by one workgroup of kCinBlock threads.
+// Density key of logical tile t: Gaussian centres counted in the 3 x 3
+// cells around the tile's cell (0 outside the image).
+__device__ __forceinline__ uint32_t density_key(const TileSort& ts, const uint32_t* dl, uint32_t t) {
+    const uint32_t st = t / 64u, local = t % 64u;
+    const int tx = (int)(st % (uint32_t)ts.blocks_x) * 8 + (int)(local % 8u);
+    const int ty = (int)(st / (uint32_t)ts.blocks_x) * 8 + (int)(local / 8u);
+    if (tx >= ts.tiles_w || ty >= ts.tiles_h) return 0;
+    const int cx = tx >> ts.cell_shift, cy = ty >> ts.cell_shift;
+    uint32_t k = 0;
+    for (int y = max(cy - 1, 0); y <= min(cy + 1, ts.cells_h - 1); ++y)
+        for (int x = max(cx - 1, 0); x <= min(cx + 1, ts.cells_w - 1); ++x) k += dl[y * ts.cells_w + x];
+    return k;
+}
+
+// The sort, run by one workgroup of kCinBlock threads.  Keys: the smoothed
+// durations of the previous launch, or (ts.density) the centre density
+// around each tile.
 __device__ void tile_sort_block(const TileSort& ts) {
     __shared__ uint32_t cls[kSchedClasses];
+    __shared__ uint32_t dl[kDensCells];
     const uint32_t* __restrict__ cost = ts.cost;
     uint32_t* __restrict__ smooth = ts.smooth;
     uint32_t* __restrict__ order = ts.order;
@@ -193,12 +219,28 @@ __device__ void tile_sort_block(const TileSort& ts) {
     const int reset = ts.reset;
     const uint32_t tid = threadIdx.x;
     for (uint32_t k = tid; k < (uint32_t)kSchedClasses; k += blockDim.x) cls[k] = 0;
+    if (ts.density) {
+        const uint32_t nc = (uint32_t)(ts.cells_w * ts.cells_h);
+        for (uint32_t k = tid; k < nc; k += blockDim.x) {
+            dl[k] = ts.dens[k];
+            ts.dens[k] = 0; // ready for the next estimate
+        }
+        __syncthreads();
+        constexpr uint32_t kUd = 8;
+        for (uint32_t b0 = 0; b0 < n; b0 += kUd * blockDim.x) {
+#pragma unroll
+            for (uint32_t j = 0; j < kUd; ++j) {
+                const uint32_t t = b0 + j * blockDim.x + tid;
+                if (t < n) atomicAdd(&cls[sched_class(density_key(ts, dl, t))], 1u);
+            }
+        }
+    }
     __syncthreads();
     // eight elements per thread and step, loaded before any is used, so a
     // step waits for one memory round trip rather than eight
     constexpr uint32_t kU = 8;
     const uint32_t stride = kU * blockDim.x;
-    for (uint32_t b0 = 0; b0 < n; b0 += stride) {
+    for (uint32_t b0 = 0; !ts.density && b0 < n; b0 += stride) {
         uint32_t c[kU], e[kU];
 #pragma unroll
         for (uint32_t j = 0; j < kU; ++j) {
@@ -244,7 +286,7 @@ __device__ void tile_sort_block(const TileSort& ts) {
 #pragma unroll
         for (uint32_t j = 0; j < kU; ++j) {
             const uint32_t t = b0 + j * blockDim.x + tid;
-            v[j] = t < n ? smooth[t] : 0u;
+            v[j] = t < n ? (ts.density ? density_key(ts, dl, t) : smooth[t]) : 0u;
         }
 #pragma unroll
         for (uint32_t j = 0; j < kU; ++j) {
@@ -439,6 +481,51 @@ __global__ __launch_bounds__(256) void k_copy(float4* __restrict__ dst, const fl
 // a.d >= Q and the f32 a.d >= q = Q - 4eps|a|.  q is formed in double and
 // rounded down; otherwise (origin near or inside the sphere, or r = inf) the
 // record accepts every direction.
+// First-launch estimate of the tile costs: each Gaussian centre in front of
+// the camera is projected to the image and counted in its density cell.  The
+// sort keys each tile by the centres in the 3 x 3 cells around it: where many
+// Gaussians project, rays meet many leaves and candidates.
+struct DensityArgs {
+    float eye[3];
+    float n[3];     // u x v (any length)
+    float sn;       // (start - eye) . n
+    float se[3];    // start - eye
+    float pu[3];    // u / (|u|^2 step_x): pixel x of a point on the plane
+    float pv[3];    // v / (|v|^2 step_y): minus pixel y
+    float cell_px;  // 1 / cell edge in pixels
+    int W, H, cells_w, cells_h;
+};
+
+// Counts go to a per-workgroup histogram in LDS first: the centres of a
+// dense cluster fall into few cells, and one global atomic per centre on
+// those cells serialises (177 us for 1M centres); from LDS each cell costs
+// one global atomic per workgroup.
+constexpr uint32_t kDensBlock = 1024;
+
+__global__ __launch_bounds__(kDensBlock) void k_density(const float4* __restrict__ gsph, uint64_t n_g, DensityArgs d,
+                                                        uint32_t* __restrict__ dens) {
+    __shared__ uint32_t hist[kDensCells];
+    const uint32_t nc = (uint32_t)(d.cells_w * d.cells_h);
+    for (uint32_t k = threadIdx.x; k < nc; k += kDensBlock) hist[k] = 0;
+    __syncthreads();
+    for (uint64_t i = (uint64_t)blockIdx.x * kDensBlock + threadIdx.x; i < n_g; i += (uint64_t)gridDim.x * kDensBlock) {
+        const float4 c = gsph[i];
+        const float rx = c.x - d.eye[0], ry = c.y - d.eye[1], rz = c.z - d.eye[2];
+        const float t = (rx * d.n[0] + ry * d.n[1] + rz * d.n[2]) / d.sn;
+        if (!(t > 0.0f)) continue;
+        const float inv = 1.0f / t;
+        const float qx = rx * inv - d.se[0], qy = ry * inv - d.se[1], qz = rz * inv - d.se[2];
+        const float px = qx * d.pu[0] + qy * d.pu[1] + qz * d.pu[2];
+        const float py = -(qx * d.pv[0] + qy * d.pv[1] + qz * d.pv[2]);
+        if (!(px >= 0.0f && px < (float)d.W && py >= 0.0f && py < (float)d.H)) continue;
+        const int cx = min((int)(px * d.cell_px), d.cells_w - 1), cy = min((int)(py * d.cell_px), d.cells_h - 1);
+        atomicAdd(&hist[cy * d.cells_w + cx], 1u);
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < nc; k += kDensBlock)
+        if (hist[k]) atomicAdd(&dens[k], hist[k]);
+}
+
 // SORT: the first workgroup (dispatched first, so it starts at once) sorts
 // the tile order of the coming view launch instead (tile_sort_block) while
 // the others stream the cone records, so the sort adds no time of its own.
@@ -596,6 +683,7 @@ struct kdgrt_scene {
     void* d_sgeom = nullptr;
     void* d_shade = nullptr;
     void* d_lsph = nullptr;
+    void* d_gsph = nullptr; // bound spheres per Gaussian (centres for the tile-cost estimate)
     // Cone records of the plain view launches, one table per scene: launches
     // on one stream run in order; a launch on another stream first waits for
     // the last launch that used the table (cone_ev).
@@ -937,7 +1025,7 @@ kdgrt_status kdgrt_framebuffer_to_u8(kdgrt_context* ctx, const float* fb, int wi
 uint64_t kdgrt_scene_bytes_estimate(uint32_t n_nodes, uint64_t n_offsets, uint64_t n_gaussians) {
     auto r = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
     return r(8ull * n_nodes) + r(48ull * n_nodes) + r(4ull * n_offsets) + r(80ull * std::max<uint64_t>(1, n_offsets)) +
-           r(192ull * n_gaussians) + 2 * r(16ull * (n_offsets + kLsphPad));
+           r(192ull * n_gaussians) + 2 * r(16ull * (n_offsets + kLsphPad)) + r(16ull * n_gaussians);
 }
 
 kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uint32_t n_nodes,
@@ -1007,6 +1095,7 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
         if (sc->d_sgeom) hipFree(sc->d_sgeom);
         if (sc->d_shade) hipFree(sc->d_shade);
         if (sc->d_lsph) hipFree(sc->d_lsph);
+        if (sc->d_gsph) hipFree(sc->d_gsph);
         if (sc->d_vcone) hipFree(sc->d_vcone);
         if (sc->cone_ev) hipEventDestroy(sc->cone_ev);
         delete sc;
@@ -1018,6 +1107,9 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
     if ((e = hipMalloc(&sc->d_offs, std::max<uint64_t>(4, 4ull * n_offsets))) != hipSuccess) return fail(e, "offsets");
     if ((e = hipMalloc(&sc->d_shade, 192ull * n_g)) != hipSuccess) return fail(e, "sh");
     if ((e = hipMalloc(&sc->d_lsph, 16ull * lsph.size())) != hipSuccess) return fail(e, "leaf spheres");
+    if ((e = hipMalloc(&sc->d_gsph, 16ull * std::max<uint64_t>(1, n_g))) != hipSuccess) return fail(e, "spheres");
+    if (n_g && (e = hipMemcpy(sc->d_gsph, sph.data(), 16ull * n_g, hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(e, "copy");
     if ((e = hipMalloc(&sc->d_vcone, 16ull * lsph.size())) != hipSuccess) return fail(e, "view cones");
     if ((e = hipEventCreateWithFlags(&sc->cone_ev, hipEventDisableTiming)) != hipSuccess) return fail(e, "event");
     if ((e = hipMemcpy(sc->d_lsph, lsph.data(), 16ull * lsph.size(), hipMemcpyHostToDevice)) != hipSuccess)
@@ -1098,6 +1190,7 @@ kdgrt_status kdgrt_scene_destroy(kdgrt_scene* sc) {
     hipFree(sc->d_sgeom);
     hipFree(sc->d_shade);
     hipFree(sc->d_lsph);
+    hipFree(sc->d_gsph);
     hipFree(sc->d_vcone);
     hipFree(sc->d_sched);
     hipEventDestroy(sc->cone_ev);
@@ -1164,11 +1257,14 @@ static uint32_t view_tiles(const ViewArgs& a) {
 }
 
 // Whether the previous view of the scene is close enough to this one for its
-// tile durations to order this launch: same size and stack depth, image axes
-// and centre ray within half a degree, eye moved by less than 1 % of its
-// distance to the scene box centre, pixel footprint within 1 %.  (Orbiting
-// the 1M-Gaussian scene by 0.25 degrees per frame keeps about half of the
-// static view's gain; at 1 degree per frame none is left.)
+// tile durations to order this launch: same kernel, size and stack depth,
+// image axes and centre ray within 0.05 degrees, eye moved by less than
+// 0.1 % of its distance to the scene box centre, pixel footprint within
+// 0.1 %.  Any other launch is ordered by the projected density of Gaussian
+// centres, which measured better as soon as the camera moves: orbiting the
+// 1M-Gaussian scene by 0.25 degrees per frame, durations 1055 Mrays/s,
+// density about 1125; for a camera that stays put, durations 1145, density
+// about 1125.
 static bool sched_similar(const kdgrt_scene* sc, const ViewArgs& a, int kind) {
     if (!sc->sched_valid || sc->sched_kind != kind || sc->sched_w != a.W || sc->sched_h != a.H ||
         sc->sched_depth != a.depth)
@@ -1189,7 +1285,7 @@ static bool sched_similar(const kdgrt_scene* sc, const ViewArgs& a, int kind) {
         }
         return (xx > 0 && yy > 0) ? xy / std::sqrt(xx * yy) : -1.0;
     };
-    const double cos_lim = 0.99996192306417; // cos(0.5 degrees)
+    const double cos_lim = 0.99999961922; // cos(0.05 degrees)
     double r0[3], r1[3], u0[3], u1[3], v0[3], v1[3], de2 = 0, dist2 = 0;
     centre_ray(p, r0);
     centre_ray(c, r1);
@@ -1204,25 +1300,29 @@ static bool sched_similar(const kdgrt_scene* sc, const ViewArgs& a, int kind) {
         dist2 += dc * dc;
     }
     if (cos_angle(r0, r1) < cos_lim || cos_angle(u0, u1) < cos_lim || cos_angle(v0, v1) < cos_lim) return false;
-    if (de2 > 1e-4 * dist2) return false;
+    if (de2 > 1e-6 * dist2) return false;
     auto footprint = [](const float* w, float step) {
         return std::sqrt((double)w[0] * w[0] + (double)w[1] * w[1] + (double)w[2] * w[2]) * std::fabs((double)step);
     };
     const double fx0 = footprint(p.u, p.step_x), fx1 = footprint(c.u, c.step_x);
     const double fy0 = footprint(p.v, p.step_y), fy1 = footprint(c.v, c.step_y);
-    return std::fabs(fx1 - fx0) <= 0.01 * fx0 && std::fabs(fy1 - fy0) <= 0.01 * fy0;
+    return std::fabs(fx1 - fx0) <= 1e-3 * fx0 && std::fabs(fy1 - fy0) <= 1e-3 * fy0;
 }
 
 // Schedule state for a launch of n logical tiles by the kernel of `kind`
-// (inside with_cone_table).  When the scene's last launch was of a view
-// close to this one, `ts` asks k_cone_prep to turn that launch's tile
-// durations into this launch's order (a.order); otherwise the launch runs in
-// raster order and no sort is run.  Either way the launch records its own
-// durations (a.cost).  `use` is false when the schedule is off or the launch
-// is too large for the one-workgroup sort.
-static kdgrt_status sched_before(kdgrt_scene* sc, ViewArgs& a, uint32_t n, int kind, bool& use, TileSort& ts) {
+// (inside with_cone_table, before k_cone_prep on stream s).  When the
+// scene's last launch was of a view close to this one, `ts` asks k_cone_prep
+// to order this launch by that launch's tile durations; otherwise k_density
+// counts the Gaussian centres that project into each cell of the image and
+// `ts` asks for an order by that density.  Either way the launch records its
+// own durations (a.cost).  `use` is false when the schedule is off or the
+// launch is too large for the one-workgroup sort.
+static kdgrt_status sched_before(kdgrt_scene* sc, ViewArgs& a, uint32_t n, int kind, hipStream_t s, bool& use,
+                                 TileSort& ts) {
     use = sc->sched_on && n <= kSchedMaxTiles;
-    ts.on = 0;
+    ts = TileSort{};
+    a.order = nullptr;
+    a.cost = nullptr;
     if (!use) return KDGRT_OK;
     if (n > sc->sched_cap) {
         // earlier launches of this scene (ordered by cone_ev) may still use
@@ -1232,21 +1332,75 @@ static kdgrt_status sched_before(kdgrt_scene* sc, ViewArgs& a, uint32_t n, int k
         sc->d_sched = nullptr;
         sc->sched_cap = 0;
         sc->sched_valid = false;
-        HIP_TRY(hipMalloc((void**)&sc->d_sched, 3ull * n * sizeof(uint32_t)));
+        HIP_TRY(hipMalloc((void**)&sc->d_sched, (3ull * n + kDensCells) * sizeof(uint32_t)));
+        HIP_TRY(hipMemset(sc->d_sched + 3ull * n, 0, kDensCells * sizeof(uint32_t)));
         sc->sched_cap = n;
     }
     uint32_t* cost = sc->d_sched;
     uint32_t* smooth = sc->d_sched + sc->sched_cap;
     uint32_t* order = sc->d_sched + 2ull * sc->sched_cap;
-    a.order = nullptr;
+    uint32_t* dens = sc->d_sched + 3ull * sc->sched_cap;
+    ts.cost = cost;
+    ts.smooth = smooth;
+    ts.order = order;
+    ts.dens = dens;
+    ts.n = n;
+    a.cost = cost;
     if (sched_similar(sc, a, kind)) {
-        ts = TileSort{cost, smooth, order, n, sc->sched_fresh ? 1 : 0, 1};
+        ts.reset = sc->sched_fresh ? 1 : 0;
+        ts.on = 1;
         sc->sched_fresh = false;
         a.order = order;
-    } else {
-        sc->sched_fresh = true; // the smoothed durations belong to another view
+        return KDGRT_OK;
     }
-    a.cost = cost;
+    sc->sched_fresh = true; // the smoothed durations belong to another view
+    // density estimate: tiles of te pixels, cells of 2^cell_shift tiles
+    const int te = a.tile4 ? 4 : 8;
+    const int tw = (a.W + te - 1) / te, th = (a.H + te - 1) / te;
+    int cs = 1;
+    while ((uint64_t)((tw + (1 << cs) - 1) >> cs) * (uint64_t)((th + (1 << cs) - 1) >> cs) > kDensCells) ++cs;
+    const CamInfo& c = a.cam;
+    DensityArgs d;
+    const double nx = (double)c.u[1] * c.v[2] - (double)c.u[2] * c.v[1];
+    const double ny = (double)c.u[2] * c.v[0] - (double)c.u[0] * c.v[2];
+    const double nz = (double)c.u[0] * c.v[1] - (double)c.u[1] * c.v[0];
+    double sn = 0, uu = 0, vv = 0;
+    for (int k = 0; k < 3; ++k) {
+        d.eye[k] = c.eye[k];
+        d.se[k] = c.start[k] - c.eye[k];
+        uu += (double)c.u[k] * c.u[k];
+        vv += (double)c.v[k] * c.v[k];
+    }
+    d.n[0] = (float)nx;
+    d.n[1] = (float)ny;
+    d.n[2] = (float)nz;
+    sn = (double)d.se[0] * nx + (double)d.se[1] * ny + (double)d.se[2] * nz;
+    if (!(std::fabs(sn) > 0.0) || !(uu > 0.0) || !(vv > 0.0) || c.step_x == 0.0f || c.step_y == 0.0f)
+        return KDGRT_OK; // degenerate camera: raster order
+    d.sn = (float)sn;
+    for (int k = 0; k < 3; ++k) {
+        d.pu[k] = (float)(c.u[k] / (uu * c.step_x));
+        d.pv[k] = (float)(c.v[k] / (vv * c.step_y));
+    }
+    d.W = a.W;
+    d.H = a.H;
+    d.cells_w = (tw + (1 << cs) - 1) >> cs;
+    d.cells_h = (th + (1 << cs) - 1) >> cs;
+    d.cell_px = 1.0f / (float)(te << cs);
+    const uint64_t per_block = 4ull * kDensBlock; // centres per workgroup, at most 512 workgroups
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((sc->n_gaussians + per_block - 1) / per_block, 512);
+    if (sc->n_gaussians)
+        hipLaunchKernelGGL(k_density, dim3(grid), dim3(kDensBlock), 0, s, (const float4*)sc->d_gsph, sc->n_gaussians,
+                           d, dens);
+    ts.on = 1;
+    ts.density = 1;
+    ts.blocks_x = a.blocks_x;
+    ts.tiles_w = tw;
+    ts.tiles_h = th;
+    ts.cell_shift = cs;
+    ts.cells_w = d.cells_w;
+    ts.cells_h = d.cells_h;
+    a.order = order;
     return KDGRT_OK;
 }
 
@@ -1277,7 +1431,7 @@ static kdgrt_status with_cone_table(const kdgrt_scene* csc, ViewArgs& a, hipStre
     TileSort ts{};
     bool use = false;
     if (sched_kind >= 0) {
-        const kdgrt_status st = sched_before(sc, a, view_tiles(a), sched_kind, use, ts);
+        const kdgrt_status st = sched_before(sc, a, view_tiles(a), sched_kind, s, use, ts);
         if (st != KDGRT_OK) return st;
     }
     const uint64_t n = sc->n_offsets + kLsphPad;

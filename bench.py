@@ -335,8 +335,10 @@ def orbit_view(aabb, yaw_deg, w=1920, h=1080):
 
 def tile_order_legs(dev, sc, ci, aabb, W, H, out, stream, steps):
     """The plain view in raster tile order (schedule off), and a camera that
-    orbits by 0.1 and 0.25 degrees per launch with and without the schedule:
-    how much of the repeated view's gain survives motion."""
+    orbits by 0.1, 0.25 and 2 degrees per launch with and without the
+    schedule.  For a camera that stays put the order comes from the previous
+    launch's tile durations; once it moves (more than 0.05 degrees per
+    launch), from the density of Gaussian centres projected into the image."""
     def timed(cams, sched):
         sc.set_tile_schedule(sched)
         dev.render(sc, cams[0], W, H, out, stream=stream)
@@ -350,7 +352,7 @@ def tile_order_legs(dev, sc, ci, aabb, W, H, out, stream, steps):
     r = {}
     ms = timed([ci] * steps, False)
     r["raster_order"] = {"ms_per_launch": round(ms, 4), "mrays_s": round(W * H / (ms * 1e-3) / 1e6, 1)}
-    for step in (0.1, 0.25):
+    for step in (0.1, 0.25, 2.0):
         cams = [orbit_view(aabb, step * k, W, H) for k in range(steps)]
         m_sched, m_raster = timed(cams, True), timed(cams, False)
         r[f"orbit_{step}_deg_per_launch"] = {"scheduled_mrays_s": round(W * H / (m_sched * 1e-3) / 1e6, 1),

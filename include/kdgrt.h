@@ -195,16 +195,17 @@ KDGRT_API kdgrt_status kdgrt_scene_create_from_tree(kdgrt_context* ctx, const kd
                                                     int scales_inverted, kdgrt_scene** scene);
 KDGRT_API kdgrt_status kdgrt_scene_destroy(kdgrt_scene* scene);
 KDGRT_API kdgrt_status kdgrt_scene_device_bytes(const kdgrt_scene* scene, uint64_t* bytes);
-/* Frame-to-frame tile schedule of kdgrt_render and kdgrt_render_hybrid, on
- * by default.  Each view launch records how long each of its tiles ran; the
- * next launch by the same kernel of the same scene with the same size and
- * stack depth, and a camera within half a degree (eye within 1 % of its
- * distance to the scene box centre), starts the tiles that ran longest
- * first, which shortens the launch's tail.  The order changes no result:
+/* Tile schedule of kdgrt_render and kdgrt_render_hybrid, on by default.  A
+ * view launch takes its tiles longest-first instead of in raster order, which
+ * shortens the launch's tail.  When the scene's previous launch was of the
+ * same view (same kernel, size and stack depth; camera within 0.05 degrees,
+ * eye within 0.1 % of its distance to the scene box centre), the order comes
+ * from the tile durations that launch recorded; otherwise from the number of
+ * Gaussian centres that project near each tile.  The order changes no result:
  * every pixel is computed by the same code.  Off (enable = 0), every launch
  * runs in raster order, as the reference's 2-D grid does
- * (cudaRenderer.cu:3169-3180).  The schedule state (12 B per tile) is
- * allocated on first use and held by the scene. */
+ * (cudaRenderer.cu:3169-3180).  The schedule state (12 B per tile and 32 KB)
+ * is allocated on first use and held by the scene. */
 KDGRT_API kdgrt_status kdgrt_scene_set_tile_schedule(kdgrt_scene* scene, int enable);
 /* Rough device footprint of a scene before creating it. */
 KDGRT_API uint64_t kdgrt_scene_bytes_estimate(uint32_t n_nodes, uint64_t n_offsets, uint64_t n_gaussians);
