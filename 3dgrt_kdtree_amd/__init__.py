@@ -533,9 +533,10 @@ class Scene:
         return b.value
 
     def set_tile_schedule(self, enable):
-        """Frame-to-frame tile order of Device.render (on by default): later
-        launches of a nearly unchanged view start the tiles that ran longest
-        first.  Results are the same either way."""
+        """Tile order of Device.render / render_hybrid (on by default): tiles
+        start longest-first, by the previous launch's durations when the
+        camera stays put, else by the projected density of Gaussian centres.
+        Off, launches run in raster order.  Results are the same either way."""
         check(lib().kdgrt_scene_set_tile_schedule(self.h, C.c_int(1 if enable else 0)))
 
     def free(self):

@@ -1317,24 +1317,32 @@ static bool sched_similar(const kdgrt_scene* sc, const ViewArgs& a, int kind) {
 // `ts` asks for an order by that density.  Either way the launch records its
 // own durations (a.cost).  `use` is false when the schedule is off or the
 // launch is too large for the one-workgroup sort.
-static kdgrt_status sched_before(kdgrt_scene* sc, ViewArgs& a, uint32_t n, int kind, hipStream_t s, bool& use,
-                                 TileSort& ts) {
-    use = sc->sched_on && n <= kSchedMaxTiles;
+// Schedule buffers for launches of up to n logical tiles (with cone_mu held).
+static kdgrt_status sched_grow(kdgrt_scene* sc, uint32_t n) {
+    if (n <= sc->sched_cap) return KDGRT_OK;
+    // earlier launches of this scene (ordered by cone_ev) may still use the
+    // old buffer
+    if (sc->cone_used) HIP_TRY(hipEventSynchronize(sc->cone_ev));
+    HIP_TRY(hipFree(sc->d_sched));
+    sc->d_sched = nullptr;
+    sc->sched_cap = 0;
+    sc->sched_valid = false;
+    HIP_TRY(hipMalloc((void**)&sc->d_sched, (3ull * n + kDensCells) * sizeof(uint32_t)));
+    HIP_TRY(hipMemset(sc->d_sched + 3ull * n, 0, kDensCells * sizeof(uint32_t)));
+    sc->sched_cap = n;
+    return KDGRT_OK;
+}
+
+static kdgrt_status sched_before(kdgrt_scene* sc, ViewArgs& a, uint32_t n, int kind, bool may_alloc, hipStream_t s,
+                                 bool& use, TileSort& ts) {
+    use = sc->sched_on && n <= kSchedMaxTiles && (may_alloc || n <= sc->sched_cap);
     ts = TileSort{};
     a.order = nullptr;
     a.cost = nullptr;
     if (!use) return KDGRT_OK;
     if (n > sc->sched_cap) {
-        // earlier launches of this scene (ordered by cone_ev) may still use
-        // the old buffer
-        if (sc->cone_used) HIP_TRY(hipEventSynchronize(sc->cone_ev));
-        HIP_TRY(hipFree(sc->d_sched));
-        sc->d_sched = nullptr;
-        sc->sched_cap = 0;
-        sc->sched_valid = false;
-        HIP_TRY(hipMalloc((void**)&sc->d_sched, (3ull * n + kDensCells) * sizeof(uint32_t)));
-        HIP_TRY(hipMemset(sc->d_sched + 3ull * n, 0, kDensCells * sizeof(uint32_t)));
-        sc->sched_cap = n;
+        const kdgrt_status st = sched_grow(sc, n);
+        if (st != KDGRT_OK) return st;
     }
     uint32_t* cost = sc->d_sched;
     uint32_t* smooth = sc->d_sched + sc->sched_cap;
@@ -1421,17 +1429,19 @@ static void sched_after(kdgrt_scene* sc, const ViewArgs& a, int kind) {
 extern "C++" {
 // sched_kind >= 0: the launch records tile durations and follows the
 // scene's tile schedule for that kernel (0 plain, 1 hybrid); the tile sort,
-// when one is due, runs inside k_cone_prep.
+// when one is due, runs inside k_cone_prep.  Without may_alloc a launch whose
+// tiles exceed the schedule buffers runs in raster order instead of growing
+// them.
 template <class Launch>
 static kdgrt_status with_cone_table(const kdgrt_scene* csc, ViewArgs& a, hipStream_t s, int sched_kind,
-                                    Launch&& launch) {
+                                    bool may_alloc, Launch&& launch) {
     kdgrt_scene* sc = const_cast<kdgrt_scene*>(csc);
     std::lock_guard<std::mutex> lock(sc->cone_mu);
     if (sc->cone_used && sc->cone_stream != s) HIP_TRY(hipStreamWaitEvent(s, sc->cone_ev, 0));
     TileSort ts{};
     bool use = false;
     if (sched_kind >= 0) {
-        const kdgrt_status st = sched_before(sc, a, view_tiles(a), sched_kind, s, use, ts);
+        const kdgrt_status st = sched_before(sc, a, view_tiles(a), sched_kind, may_alloc, s, use, ts);
         if (st != KDGRT_OK) return st;
     }
     const uint64_t n = sc->n_offsets + kLsphPad;
@@ -1457,18 +1467,22 @@ static kdgrt_status with_cone_table(const kdgrt_scene* csc, ViewArgs& a, hipStre
 } // extern "C++"
 
 typedef void (*ViewKernel)(ViewArgs);
-// sched: record tile durations and follow the scene's tile schedule (the
-// plain kernel only; debug and profile launches keep the raster order).
+// Tile order of a plain launch: raster (debug, profile and warm-up
+// launches), the scene's schedule, or the schedule only within the buffers
+// the scene already has (no allocation, the serving loop).
+enum SchedUse { kSchedNone, kSchedFull, kSchedNoAlloc };
+
 static kdgrt_status launch_view(const kdgrt_scene* csc, ViewArgs& a, ViewKernel kern, hipStream_t s,
-                                bool sched = false) {
-    return with_cone_table(csc, a, s, sched ? 0 : -1, [&](kdgrt_scene*) -> kdgrt_status {
+                                SchedUse sched = kSchedNone) {
+    return with_cone_table(csc, a, s, sched == kSchedNone ? -1 : 0, sched == kSchedFull,
+                           [&](kdgrt_scene*) -> kdgrt_status {
         hipLaunchKernelGGL(kern, dim3(view_tiles(a)), dim3(64), stack_lds_bytes(a.depth, 64), s, a);
         return KDGRT_OK;
     });
 }
 
 static kdgrt_status render_plain(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgrt_camera_info* cam, int W,
-                                 int H, int depth, int fmt, void* out, void* stream, bool sched) {
+                                 int H, int depth, int fmt, void* out, void* stream, SchedUse sched) {
     kdgrt_status st = check_view(ctx, sc, cam, W, H, depth, out);
     if (st != KDGRT_OK) return st;
     if (fmt != KDGRT_OUT_RGBT_F32 && fmt != KDGRT_OUT_RGB_F32)
@@ -1483,14 +1497,22 @@ static kdgrt_status render_plain(kdgrt_context* ctx, const kdgrt_scene* sc, cons
 
 kdgrt_status kdgrt_render(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgrt_camera_info* cam, int W, int H,
                           int depth, int fmt, void* out, void* stream) {
-    return render_plain(ctx, sc, cam, W, H, depth, fmt, out, stream, true);
+    return render_plain(ctx, sc, cam, W, H, depth, fmt, out, stream, kSchedFull);
 }
 
 extern "C++" {
 namespace kdgrt {
-kdgrt_status render_unscheduled(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgrt_camera_info* cam, int W, int H,
-                                int depth, void* out, hipStream_t stream) {
-    return render_plain(ctx, sc, cam, W, H, depth, KDGRT_OUT_RGBT_F32, out, stream, false);
+kdgrt_status render_served(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgrt_camera_info* cam, int W, int H,
+                           int depth, void* out, hipStream_t stream) {
+    return render_plain(ctx, sc, cam, W, H, depth, KDGRT_OUT_RGBT_F32, out, stream, kSchedNoAlloc);
+}
+
+kdgrt_status reserve_tile_schedule(kdgrt_scene* sc, int max_w, int max_h) {
+    if (!sc || max_w <= 0 || max_h <= 0) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "reserve_tile_schedule");
+    hipSetDevice(sc->ctx->device);
+    const uint64_t n = interleave_grid<64>((uint64_t)((max_w + 63) / 64) * (uint64_t)((max_h + 63) / 64) * 64);
+    std::lock_guard<std::mutex> lock(sc->cone_mu);
+    return sched_grow(sc, (uint32_t)std::min<uint64_t>(n, kSchedMaxTiles));
 }
 } // namespace kdgrt
 } // extern "C++"
@@ -1561,7 +1583,7 @@ kdgrt_status kdgrt_render_hybrid(kdgrt_context* ctx, const kdgrt_scene* sc, cons
     }
     fill_view_args(ha.v, sc, cam, W, H, depth, out);
     const hipStream_t s = (hipStream_t)stream;
-    return with_cone_table(sc, ha.v, s, 1, [&](kdgrt_scene*) -> kdgrt_status {
+    return with_cone_table(sc, ha.v, s, 1, true, [&](kdgrt_scene*) -> kdgrt_status {
         hipLaunchKernelGGL(k_hybrid<kViewK>, dim3(view_tiles(ha.v)), dim3(64), stack_lds_bytes(depth, 64), s, ha);
         return KDGRT_OK;
     });
@@ -1613,7 +1635,7 @@ kdgrt_status kdgrt_warm_up(kdgrt_context* ctx, const kdgrt_scene* sc, void* stre
     kdgrt_camera_info ci;
     kdgrt_camera_make(&cam, 64, 64, &ci);
     // raster order: the warm-up view leaves the scene's tile schedule alone
-    kdgrt_status st = render_plain(ctx, sc, &ci, 64, 64, 12, KDGRT_OUT_RGBT_F32, tmp, stream, false);
+    kdgrt_status st = render_plain(ctx, sc, &ci, 64, 64, 12, KDGRT_OUT_RGBT_F32, tmp, stream, kSchedNone);
     hipStreamSynchronize((hipStream_t)stream);
     hipFree(tmp);
     return st;

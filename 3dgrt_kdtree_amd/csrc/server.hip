@@ -45,6 +45,10 @@ using kdgrt::set_error;
 
 namespace {
 
+// Tile-schedule buffers reserved per scene: views up to 4096 x 4096 pixels
+// (3 MB per scene) follow the tile order; larger ones run in raster order.
+constexpr int kServeMaxEdge = 4096;
+
 enum ReqState { kPending = 0, kAdmitted = 1, kDone = 2, kCancelled = 3, kFailed = 4 };
 
 struct Req {
@@ -266,11 +270,10 @@ bool launch_one(kdgrt_server* s) {
             // comparison mode, or a batch of one view: the one-view kernel
             // (its per-view cone pre-test makes it the faster path for a
             // single view); it still runs on the slot's stream and overlaps
-            // with the other slots.  Raster tile order: the frame-to-frame
-            // schedule allocates its state on first use of a size, which the
-            // launch path must not do.
-            stt = kdgrt::render_unscheduled(s->ctx, views[0].scene, &views[0].cam, views[0].width, views[0].height,
-                                            s->cfg.tuning.stack_depth, views[0].out_rgbt, st);
+            // with the other slots.  The tile order uses the schedule buffers
+            // reserved when the scene was added, so nothing is allocated here.
+            stt = kdgrt::render_served(s->ctx, views[0].scene, &views[0].cam, views[0].width, views[0].height,
+                                       s->cfg.tuning.stack_depth, views[0].out_rgbt, st);
         } else {
             uint64_t rays = 0;
             stt = kdgrt::slot_launch(s->slots[slot], views.data(), (int)views.size(), s->cfg.tuning, st, &rays);
@@ -351,6 +354,11 @@ kdgrt_status kdgrt_server_add_scene(kdgrt_server* s, const kdgrt_tree* tree, con
     if (!s->sim) {
         kdgrt_status st = kdgrt_scene_create_from_tree(s->ctx, tree, g, n_g, scales_inverted, &sc);
         if (st != KDGRT_OK) return st;
+        st = kdgrt::reserve_tile_schedule(sc, kServeMaxEdge, kServeMaxEdge);
+        if (st != KDGRT_OK) {
+            kdgrt_scene_destroy(sc);
+            return st;
+        }
     } else {
         kdgrt_status st = kdgrt::validate_tree(tree->nodes.data(), (uint32_t)tree->nodes.size(),
                                                tree->offsets.size(), tree->offsets.data(), n_g);
@@ -375,6 +383,8 @@ kdgrt_status kdgrt_server_add_resident_scene(kdgrt_server* s, kdgrt_scene* scene
     if (st != KDGRT_OK) return st;
     if (s->scene_bytes + bytes > s->cfg.hbm_budget_bytes)
         return set_error(KDGRT_ERR_CAPACITY, "server_add_resident_scene: scene does not fit the HBM budget");
+    st = kdgrt::reserve_tile_schedule(scene, kServeMaxEdge, kServeMaxEdge);
+    if (st != KDGRT_OK) return st;
     const int32_t id = s->next_scene++;
     s->scenes[id] = {scene, bytes};
     s->scene_owned[id] = false;
