@@ -146,3 +146,21 @@ def test_schedule_across_streams(dev):
     for st in streams:
         K.check(K.lib().kdgrt_stream_destroy(dev.h, st))
     sc.free()
+
+
+@pytest.mark.parametrize("wh", [(1, 1), (13, 7), (65, 33), (8, 200), (300, 9), (2000, 24)])
+def test_schedule_partial_tiles(dev, wh):
+    """Sizes that leave partial tiles, down to one pixel: the first launch
+    (ordered by centre density) and a repeat (ordered by durations) equal the
+    raster-order debug kernel's image."""
+    w, h = wh
+    hs = S.uniform_10k()
+    ci = S.cam_c0(w, h)
+    sc = dev.scene(hs.tree, hs.g)
+    ref = dev.render_debug(sc, ci, w, h)["rgbt"]
+    for k in range(2):
+        print(f"[schedule] {w}x{h} launch {k}", flush=True)
+        img = _frame(dev, sc, ci, w, h)
+        assert not np.isnan(img).any(), f"{w}x{h} launch {k}: pixels left unwritten"
+        assert np.array_equal(img, ref), f"{w}x{h} launch {k} differs"
+    sc.free()
