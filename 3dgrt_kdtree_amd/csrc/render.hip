@@ -451,6 +451,7 @@ struct WorkItem { // one view or one explicit ray set
     const float* ro;
     const float* rd;
     uint64_t n_rays;
+    const uint32_t* perm; // ray sets: processing order (ray indices), or null for list order
 };
 
 // Tile queues: the tile space is cut into groups of kQueueGroup consecutive
@@ -660,8 +661,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 4 
                 make_ray(it.cam, x, y, o, d);
                 dst = (size_t)(it.H - y - 1) * it.W + x; // framebuffer row flip
             } else {
-                const uint64_t i = (uint64_t)local * npx + p;
-                live = i < it.n_rays;
+                const uint64_t j = (uint64_t)local * npx + p;
+                live = j < it.n_rays;
+                const uint64_t i = (live && it.perm) ? (uint64_t)it.perm[j] : j;
                 o[0] = o[1] = o[2] = 0.f;
                 d[0] = d[1] = 0.f;
                 d[2] = 1.f;
@@ -760,6 +762,7 @@ struct kdgrt_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     void* graph_stream = nullptr;
+    std::vector<void*> d_perms; // ray-key order of the long ray sets
 };
 
 using kdgrt::set_error;
@@ -1625,6 +1628,37 @@ kdgrt_status kdgrt_render_hybrid(kdgrt_context* ctx, const kdgrt_scene* sc, cons
     });
 }
 
+// Ray-key order of a ray list into perm (n entries) on stream s.  The
+// scratch (keys, indices, sort temporaries) comes from the stream-ordered
+// allocator and is released on s after the sort.
+static kdgrt_status ray_order(const DevScene& sc, const float* ro, const float* rd, uint32_t n, uint32_t* perm,
+                              hipStream_t s) {
+    size_t temp = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                               (const uint32_t*)nullptr, (uint32_t*)nullptr, n, 0, kRayKeyBits, s));
+    const size_t arr = ((size_t)n * 4 + 255) & ~(size_t)255;
+    void* scratch = nullptr;
+    HIP_TRY(hipMallocAsync(&scratch, 3 * arr + temp, s));
+    char* base = (char*)scratch;
+    uint32_t* k_in = (uint32_t*)base;
+    uint32_t* k_out = (uint32_t*)(base + arr);
+    uint32_t* v_in = (uint32_t*)(base + 2 * arr);
+    void* tmp = base + 3 * arr;
+    const float* bmin = sc.bmin;
+    const float* bmax = sc.bmax;
+    const float ext[3] = {bmax[0] - bmin[0], bmax[1] - bmin[1], bmax[2] - bmin[2]};
+    const float3 lo = make_float3(bmin[0], bmin[1], bmin[2]);
+    const float3 inv = make_float3(ext[0] > 0 ? 1.0f / ext[0] : 0.0f, ext[1] > 0 ? 1.0f / ext[1] : 0.0f,
+                                   ext[2] > 0 ? 1.0f / ext[2] : 0.0f);
+    const uint32_t kgrid = (uint32_t)std::min<uint64_t>(((uint64_t)n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_ray_keys, dim3(kgrid), dim3(256), 0, s, ro, rd, n, lo, inv, k_in, v_in);
+    HIP_TRY(hipGetLastError());
+    const hipError_t e = hipcub::DeviceRadixSort::SortPairs(tmp, temp, k_in, k_out, v_in, perm, n, 0, kRayKeyBits, s);
+    HIP_TRY(hipFreeAsync(scratch, s));
+    HIP_TRY(e);
+    return KDGRT_OK;
+}
+
 static kdgrt_status launch_rays(kdgrt_context* ctx, const kdgrt_scene* sc, const float* ro, const float* rd,
                                 uint64_t n, int depth, float* out, int32_t* counters, uint64_t* hashes, bool dbg,
                                 void* stream) {
@@ -1641,30 +1675,13 @@ static kdgrt_status launch_rays(kdgrt_context* ctx, const kdgrt_scene* sc, const
     // stream-ordered allocator, so the call stays asynchronous.
     void* scratch = nullptr;
     if (n >= kRaySortMin && n <= 0x7FFFFFFFull) {
-        const uint32_t nn = (uint32_t)n;
-        size_t temp = 0;
-        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                                   (const uint32_t*)nullptr, (uint32_t*)nullptr, nn, 0,
-                                                   kRayKeyBits, s));
-        const size_t arr = ((size_t)nn * 4 + 255) & ~(size_t)255;
-        HIP_TRY(hipMallocAsync(&scratch, 4 * arr + temp, s));
-        char* base = (char*)scratch;
-        uint32_t* k_in = (uint32_t*)base;
-        uint32_t* k_out = (uint32_t*)(base + arr);
-        uint32_t* v_in = (uint32_t*)(base + 2 * arr);
-        uint32_t* v_out = (uint32_t*)(base + 3 * arr);
-        void* tmp = base + 4 * arr;
-        float3 lo, inv;
-        const float* bmin = sc->dev.bmin;
-        const float* bmax = sc->dev.bmax;
-        const float ext[3] = {bmax[0] - bmin[0], bmax[1] - bmin[1], bmax[2] - bmin[2]};
-        lo = make_float3(bmin[0], bmin[1], bmin[2]);
-        inv = make_float3(ext[0] > 0 ? 1.0f / ext[0] : 0.0f, ext[1] > 0 ? 1.0f / ext[1] : 0.0f,
-                          ext[2] > 0 ? 1.0f / ext[2] : 0.0f);
-        const uint32_t kgrid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 8192);
-        hipLaunchKernelGGL(k_ray_keys, dim3(kgrid), dim3(256), 0, s, ro, rd, nn, lo, inv, k_in, v_in);
-        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, temp, k_in, k_out, v_in, v_out, nn, 0, kRayKeyBits, s));
-        a.perm = v_out;
+        HIP_TRY(hipMallocAsync(&scratch, 4 * n, s));
+        const kdgrt_status st = ray_order(sc->dev, ro, rd, (uint32_t)n, (uint32_t*)scratch, s);
+        if (st != KDGRT_OK) {
+            hipFreeAsync(scratch, s);
+            return st;
+        }
+        a.perm = (const uint32_t*)scratch;
     }
     if (dbg) hipLaunchKernelGGL((k_rays<kViewK, true>), grid, block, lds, s, a);
     else hipLaunchKernelGGL((k_rays<kViewK, false>), grid, block, lds, s, a);
@@ -1775,6 +1792,7 @@ kdgrt_status kdgrt_batch_destroy(kdgrt_batch* b) {
     if (b->graph) hipGraphDestroy(b->graph);
     hipFree(b->d_items);
     hipFree(b->d_counters);
+    for (void* p : b->d_perms) hipFree(p);
     delete b;
     return KDGRT_OK;
 }
@@ -1856,13 +1874,31 @@ kdgrt_status batch_build(kdgrt_context* ctx, const kdgrt_view* views, int n_view
     b->tuning = t;
     b->n_tiles = (uint32_t)tiles;
     b->n_rays = rays;
-    hipError_t e = hipMalloc(&b->d_items, items.size() * sizeof(WorkItem));
+    hipError_t e = hipSuccess;
+    {
+        // long ray sets are traced in ray-key order, fixed here (the order
+        // changes no result; it stays valid whatever the rays hold later)
+        for (WorkItem& w : items) {
+            if (w.W != 0 || w.n_rays < kRaySortMin || w.n_rays > 0x7FFFFFFFull) continue;
+            void* perm = nullptr;
+            if ((e = hipMalloc(&perm, 4 * w.n_rays)) != hipSuccess) break;
+            b->d_perms.push_back(perm);
+            if (ray_order(w.sc, w.ro, w.rd, (uint32_t)w.n_rays, (uint32_t*)perm, nullptr) != KDGRT_OK) {
+                e = hipErrorUnknown;
+                break;
+            }
+            w.perm = (const uint32_t*)perm;
+        }
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+    }
+    if (e == hipSuccess) e = hipMalloc(&b->d_items, items.size() * sizeof(WorkItem));
     if (e == hipSuccess) e = hipMalloc(&b->d_counters, 64 * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemcpy(b->d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(b->d_counters, 0, 64 * sizeof(uint32_t));
     if (e != hipSuccess) {
         hipFree(b->d_items);
         hipFree(b->d_counters);
+        for (void* p : b->d_perms) hipFree(p);
         delete b;
         return set_error(KDGRT_ERR_DEVICE, std::string("batch_create: ") + hipGetErrorString(e));
     }

@@ -141,22 +141,27 @@ def test_batch_matches_plain(dev):
         views.append(K.View(sc.h, ci, w, h, out.ptr))
         outs.append((out, w, h))
         refs.append(dev.render_image(sc, ci, w, h))
-    o, d = S.random_rays(a.aabb, 5000, 9)
-    ob, db, rout = dev.upload(o), dev.upload(d), dev.alloc(5000 * 16)
-    rs = [K.RaySet(sa.h, ob.ptr, db.ptr, 5000, rout.ptr)]
+    # 70,000 rays: long enough for the batch to trace them in ray-key order
+    nr = 70000
+    o, d = S.random_rays(a.aabb, nr, 9)
+    ob, db, rout = dev.upload(o), dev.upload(d), dev.alloc(nr * 16)
+    rs = [K.RaySet(sa.h, ob.ptr, db.ptr, nr, rout.ptr)]
     rref = dev.render_rays(sa, o, d)["rgbt"]
+    # the same rays as short lists (list order) give the same colours
+    assert np.array_equal(np.concatenate([dev.render_rays(sa, o[k:k + 35000], d[k:k + 35000])["rgbt"]
+                                          for k in (0, 35000)]), rref)
     for tun in [K.default_tuning(dev), K.Tuning(16, 16, 12, 4, 4, 64), K.Tuning(16, 8, 12, 16, 8, 128)]:
         batch = K.Batch(dev, views, rs, tun)
         for use_graph in (False, True, True, True):
             for out, w, h in outs:
                 dev_zero = np.zeros((h, w, 4), np.float32)
                 out.upload(dev_zero)
-            rout.upload(np.zeros((5000, 4), np.float32))
+            rout.upload(np.zeros((nr, 4), np.float32))
             batch.run(stream=None if not use_graph else _stream(dev), use_graph=use_graph)
             dev.sync()
             for (out, w, h), ref in zip(outs, refs):
                 assert np.array_equal(out.download(np.float32, (h, w, 4)), ref)
-            assert np.array_equal(rout.download(np.float32, (5000, 4)), rref)
+            assert np.array_equal(rout.download(np.float32, (nr, 4)), rref)
         batch.free()
     sa.free()
     sb.free()
