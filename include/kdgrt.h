@@ -239,7 +239,11 @@ KDGRT_API kdgrt_status kdgrt_render_debug(kdgrt_context* ctx, const kdgrt_scene*
 KDGRT_API kdgrt_status kdgrt_render_profile(kdgrt_context* ctx, const kdgrt_scene* scene,
                                             const kdgrt_camera_info* cam, int width, int height, float* out_rgbt,
                                             uint64_t* phase_cycles, void* stream);
-/* Explicit rays (device float3 arrays, directions used as given). */
+/* Explicit rays (device float3 arrays, directions used as given).  Lists of
+ * 65,536 rays or more are traced in the order of a key (direction octant,
+ * then the Morton code of the origin in the scene box), sorted on the device
+ * with scratch from the stream-ordered allocator; results are written at the
+ * caller's indices and do not depend on the order. */
 KDGRT_API kdgrt_status kdgrt_render_rays(kdgrt_context* ctx, const kdgrt_scene* scene, const float* origins,
                                          const float* dirs, uint64_t n_rays, int stack_depth, float* out_rgbt,
                                          void* stream);
@@ -340,7 +344,9 @@ typedef struct kdgrt_tuning {
 
 KDGRT_API kdgrt_status kdgrt_tuning_default(kdgrt_context* ctx, kdgrt_tuning* t);
 /* Plan a batch: uploads the work table; the views' output buffers must stay
- * valid while the batch is run. */
+ * valid while the batch is run.  Ray sets of 65,536 rays or more get their
+ * tracing order here (as kdgrt_render_rays sorts them; waits for the
+ * device); the order changes no result, also if the rays change later. */
 KDGRT_API kdgrt_status kdgrt_batch_create(kdgrt_context* ctx, const kdgrt_view* views, int n_views,
                                           const kdgrt_ray_set* ray_sets, int n_ray_sets,
                                           const kdgrt_tuning* tuning, kdgrt_batch** batch);
