@@ -264,6 +264,16 @@ class Tree:
             pass
 
 
+def ply_read(path):
+    """Activated Gaussians of a 3DGS PLY file (kdgrt_ply_read: sigmoid opacity,
+    exp scales, normalised quaternions, k_scale), reference 256-B layout."""
+    n = C.c_uint64()
+    check(lib().kdgrt_ply_count(str(path).encode(), C.byref(n)))
+    g = np.zeros(n.value, GAUSSIAN_DTYPE)
+    check(lib().kdgrt_ply_read(str(path).encode(), _p(g), C.c_uint64(n.value), C.byref(n)))
+    return g[:n.value]
+
+
 def cameras_from_nerf_json(path, aspect):
     """Camera list of a NeRF transforms.json (kdgrt_cameras_from_nerf_json)."""
     n = C.c_int()

@@ -36,8 +36,8 @@
 //    opacity, ID), a skipped sample reads no SH data and a blended one
 //    reads 192 B;
 //  * the 15 SH basis scalars depend only on the ray direction; they are
-//    formed with the reference's double-precision products when a sample is
-//    blended (about five times per ray) instead of occupying registers.
+//    formed (float products, the reference's operator order) when a sample
+//    is blended (about five times per ray) instead of occupying registers.
 #pragma once
 
 #include <hip/hip_fp16.h>
@@ -112,34 +112,35 @@ __device__ __forceinline__ void make_ray(const CamInfo& c, int x, int y, float o
     for (int k = 0; k < 3; ++k) d[k] = w[k] * inv;
 }
 
-// SH basis scalars of eval_sh_final (cudaRenderer.cu:517-553): the double
-// constant times float operands is a double product, converted to float when
-// it meets the float3 coefficient.
+// SH basis scalars of eval_sh_final (cudaRenderer.cu:517-553).  Every
+// SH_C* constant carries an f suffix (cudaRenderer.h:33-48), so each factor
+// is a float product in the reference's left-to-right order: SH_C1 * y
+// rounds once, (SH_C3_0 * y) * (3xx - yy) twice.
 __device__ __forceinline__ void sh_basis(const float d[3], float s[16]) {
-    const double C1 = 0.4886025119029199, C2_0 = 1.0925484305920792, C2_1 = -1.0925484305920792,
-                 C2_2 = 0.31539156525252005, C2_3 = -1.0925484305920792, C2_4 = 0.5462742152960396,
-                 C3_0 = -0.5900435899266435, C3_1 = 2.890611442640554, C3_2 = -0.4570457994644658,
-                 C3_3 = 0.3731763325901154, C3_4 = -0.4570457994644658, C3_5 = 1.445305721320277,
-                 C3_6 = -0.5900435899266435;
+    const float C1 = 0.4886025119029199f, C2_0 = 1.0925484305920792f, C2_1 = -1.0925484305920792f,
+                C2_2 = 0.31539156525252005f, C2_3 = -1.0925484305920792f, C2_4 = 0.5462742152960396f,
+                C3_0 = -0.5900435899266435f, C3_1 = 2.890611442640554f, C3_2 = -0.4570457994644658f,
+                C3_3 = 0.3731763325901154f, C3_4 = -0.4570457994644658f, C3_5 = 1.445305721320277f,
+                C3_6 = -0.5900435899266435f;
     const float x = d[0], y = d[1], z = d[2];
     s[0] = 0.28209479177387814f;
-    s[1] = (float)(C1 * (double)y);
-    s[2] = (float)(C1 * (double)z);
-    s[3] = (float)(C1 * (double)x);
+    s[1] = C1 * y;
+    s[2] = C1 * z;
+    s[3] = C1 * x;
     const float xx = x * x, yy = y * y, zz = z * z;
     const float xy = x * y, yz = y * z, xz = x * z;
-    s[4] = (float)(C2_0 * (double)xy);
-    s[5] = (float)(C2_1 * (double)yz);
-    s[6] = (float)(C2_2 * (double)(2.0f * zz - xx - yy));
-    s[7] = (float)(C2_3 * (double)xz);
-    s[8] = (float)(C2_4 * (double)(xx - yy));
-    s[9] = (float)((C3_0 * (double)y) * (double)(3.0f * xx - yy));
-    s[10] = (float)((C3_1 * (double)xy) * (double)z);
-    s[11] = (float)((C3_2 * (double)y) * (double)(4.0f * zz - xx - yy));
-    s[12] = (float)((C3_3 * (double)z) * (double)(2.0f * zz - 3.0f * xx - 3.0f * yy));
-    s[13] = (float)((C3_4 * (double)x) * (double)(4.0f * zz - xx - yy));
-    s[14] = (float)((C3_5 * (double)z) * (double)(xx - yy));
-    s[15] = (float)((C3_6 * (double)x) * (double)(xx - 3.0f * yy));
+    s[4] = C2_0 * xy;
+    s[5] = C2_1 * yz;
+    s[6] = C2_2 * (2.0f * zz - xx - yy);
+    s[7] = C2_3 * xz;
+    s[8] = C2_4 * (xx - yy);
+    s[9] = (C3_0 * y) * (3.0f * xx - yy);
+    s[10] = (C3_1 * xy) * z;
+    s[11] = (C3_2 * y) * (4.0f * zz - xx - yy);
+    s[12] = (C3_3 * z) * (2.0f * zz - 3.0f * xx - 3.0f * yy);
+    s[13] = (C3_4 * x) * (4.0f * zz - xx - yy);
+    s[14] = (C3_5 * z) * (xx - yy);
+    s[15] = (C3_6 * x) * (xx - 3.0f * yy);
 }
 
 // Radiance of one Gaussian: the 16 coefficient float3s times the basis

@@ -113,40 +113,69 @@ template <class T> inline void quat_to_mt(const float q[4], V3<T> m[3]) {
     m[2] = {T(2) * (xz + ry), T(2) * (yz - rx), (T(1) - T(2) * (xx + yy))};
 }
 
-// Spherical-harmonic constants: double literals (cudaRenderer.h:33-48).
-const double SH_C0 = 0.28209479177387814, SH_C1 = 0.4886025119029199, SH_C2_0 = 1.0925484305920792,
-             SH_C2_1 = -1.0925484305920792, SH_C2_2 = 0.31539156525252005, SH_C2_3 = -1.0925484305920792,
-             SH_C2_4 = 0.5462742152960396, SH_C3_0 = -0.5900435899266435, SH_C3_1 = 2.890611442640554,
-             SH_C3_2 = -0.4570457994644658, SH_C3_3 = 0.3731763325901154, SH_C3_4 = -0.4570457994644658,
-             SH_C3_5 = 1.445305721320277, SH_C3_6 = -0.5900435899266435;
+// Spherical-harmonic constants (cudaRenderer.h:33-48).  The reference
+// defines every SH_C* with an f suffix, so each basis factor is a float
+// product: SH_C1 * y rounds once, and the degree-3 factors
+// (SH_C3_0 * y) * (3xx - yy) round twice (C++ evaluates a * b * c as
+// (a * b) * c).  ShC<float> holds those float literals; ShC<double> holds the
+// same decimal values as double literals for the FP64 oracle.
+template <class T> struct ShC;
+template <> struct ShC<float> {
+    static constexpr float C0 = 0.28209479177387814f, C1 = 0.4886025119029199f, C2_0 = 1.0925484305920792f,
+                           C2_1 = -1.0925484305920792f, C2_2 = 0.31539156525252005f, C2_3 = -1.0925484305920792f,
+                           C2_4 = 0.5462742152960396f, C3_0 = -0.5900435899266435f, C3_1 = 2.890611442640554f,
+                           C3_2 = -0.4570457994644658f, C3_3 = 0.3731763325901154f, C3_4 = -0.4570457994644658f,
+                           C3_5 = 1.445305721320277f, C3_6 = -0.5900435899266435f;
+};
+template <> struct ShC<double> {
+    static constexpr double C0 = 0.28209479177387814, C1 = 0.4886025119029199, C2_0 = 1.0925484305920792,
+                            C2_1 = -1.0925484305920792, C2_2 = 0.31539156525252005, C2_3 = -1.0925484305920792,
+                            C2_4 = 0.5462742152960396, C3_0 = -0.5900435899266435, C3_1 = 2.890611442640554,
+                            C3_2 = -0.4570457994644658, C3_3 = 0.3731763325901154, C3_4 = -0.4570457994644658,
+                            C3_5 = 1.445305721320277, C3_6 = -0.5900435899266435;
+};
 
-// eval_sh_final(3, dir, g) (cudaRenderer.cu:487-558).  A double constant
-// times a float scalar is a double product; it reaches the float3 operator*
-// converted to float (the only viable overload is operator*(float, float3)).
+// The 16 basis scalars of eval_sh_final(3, dir, g) (cudaRenderer.cu:517-553)
+// as the reference forms them: s[0] = SH_C0; degree 1 SH_C1 * (y, z, x) with
+// the signs (-, +, -) applied in eval_sh; degree 2 SH_C2_i * poly; degree 3
+// (SH_C3_i * axis) * poly, or (SH_C3_1 * xy) * z.
+template <class T> inline void sh_basis(V3<T> d, T s[16]) {
+    using C = ShC<T>;
+    const T x = d.x, y = d.y, z = d.z;
+    s[0] = C::C0;
+    s[1] = C::C1 * y;
+    s[2] = C::C1 * z;
+    s[3] = C::C1 * x;
+    const T xx = x * x, yy = y * y, zz = z * z;
+    const T xy = x * y, yz = y * z, xz = x * z;
+    s[4] = C::C2_0 * xy;
+    s[5] = C::C2_1 * yz;
+    s[6] = C::C2_2 * (T(2) * zz - xx - yy);
+    s[7] = C::C2_3 * xz;
+    s[8] = C::C2_4 * (xx - yy);
+    s[9] = C::C3_0 * y * (T(3) * xx - yy);
+    s[10] = C::C3_1 * xy * z;
+    s[11] = C::C3_2 * y * (T(4) * zz - xx - yy);
+    s[12] = C::C3_3 * z * (T(2) * zz - T(3) * xx - T(3) * yy);
+    s[13] = C::C3_4 * x * (T(4) * zz - xx - yy);
+    s[14] = C::C3_5 * z * (xx - yy);
+    s[15] = C::C3_6 * x * (xx - T(3) * yy);
+}
+
+// eval_sh_final(3, dir, g) (cudaRenderer.cu:487-558): rad = SH_C0 * c0, then
+// rad - s1 c1 + s2 c2 - s3 c3, then + s_i c_i for i = 4..15, left to right
+// (operator*(float, float3) scales each component), + 0.5, clamped below at 0.
 template <class T> inline V3<T> eval_sh(V3<T> d, const Gauss256& g) {
     V3<T> c[16];
     c[0] = {T(g.f_dc[0]), T(g.f_dc[1]), T(g.f_dc[2])};
     for (int i = 0; i < 15; ++i) c[i + 1] = {T(g.f_rest[i]), T(g.f_rest[15 + i]), T(g.f_rest[30 + i])};
-    auto S = [](double v) { return T(v); };
-    V3<T> rad = muls(c[0], S(SH_C0));
-    const T x = d.x, y = d.y, z = d.z;
-    rad = sub(rad, muls(c[1], S(SH_C1 * y)));
-    rad = add(rad, muls(c[2], S(SH_C1 * z)));
-    rad = sub(rad, muls(c[3], S(SH_C1 * x)));
-    const T xx = x * x, yy = y * y, zz = z * z;
-    const T xy = x * y, yz = y * z, xz = x * z;
-    rad = add(rad, muls(c[4], S(SH_C2_0 * xy)));
-    rad = add(rad, muls(c[5], S(SH_C2_1 * yz)));
-    rad = add(rad, muls(c[6], S(SH_C2_2 * (T(2) * zz - xx - yy))));
-    rad = add(rad, muls(c[7], S(SH_C2_3 * xz)));
-    rad = add(rad, muls(c[8], S(SH_C2_4 * (xx - yy))));
-    rad = add(rad, muls(c[9], S(SH_C3_0 * y * (T(3) * xx - yy))));
-    rad = add(rad, muls(c[10], S(SH_C3_1 * xy * z)));
-    rad = add(rad, muls(c[11], S(SH_C3_2 * y * (T(4) * zz - xx - yy))));
-    rad = add(rad, muls(c[12], S(SH_C3_3 * z * (T(2) * zz - T(3) * xx - T(3) * yy))));
-    rad = add(rad, muls(c[13], S(SH_C3_4 * x * (T(4) * zz - xx - yy))));
-    rad = add(rad, muls(c[14], S(SH_C3_5 * z * (xx - yy))));
-    rad = add(rad, muls(c[15], S(SH_C3_6 * x * (xx - T(3) * yy))));
+    T s[16];
+    sh_basis<T>(d, s);
+    V3<T> rad = muls(c[0], s[0]);
+    rad = sub(rad, muls(c[1], s[1]));
+    rad = add(rad, muls(c[2], s[2]));
+    rad = sub(rad, muls(c[3], s[3]));
+    for (int i = 4; i < 16; ++i) rad = add(rad, muls(c[i], s[i]));
     rad = add(rad, V3<T>{T(0.5f), T(0.5f), T(0.5f)});
     return {tmax(rad.x, T(0)), tmax(rad.y, T(0)), tmax(rad.z, T(0))};
 }
@@ -175,10 +204,11 @@ struct RayOut {
 
 const uint64_t kFnvBasis = 0xcbf29ce484222325ull, kFnvPrime = 0x100000001b3ull;
 
-// rayPrimIntersect, VOLUME_ISECT (cudaRenderer.cu:1069-1135).
+// rayPrimIntersect, VOLUME_ISECT (cudaRenderer.cu:1069-1135): t of the
+// closest approach in normalised space and grayDist, the squared distance
+// there; accepted when t_near <= t <= t_far and grayDist < 8.
 template <class T>
-inline void ray_prim(const V3<T>& o, const V3<T>& d, uint32_t id, const Gauss256& g, T t_near, T t_far,
-                     std::vector<HitRec<T>>& hits) {
+inline bool ray_gaussian(const V3<T>& o, const V3<T>& d, const Gauss256& g, T t_near, T t_far, T& t, T& grayDist) {
     const V3<T> ppos = {T(g.pos[0]), T(g.pos[1]), T(g.pos[2])};
     const V3<T> giscl = {T(g.scale[0]), T(g.scale[1]), T(g.scale[2])};
     const V3<T> pscale = {T(1) / giscl.x, T(1) / giscl.y, T(1) / giscl.z};
@@ -192,11 +222,17 @@ inline void ray_prim(const V3<T>& o, const V3<T>& d, uint32_t id, const Gauss256
     const V3<T> grd = safe_normalize(grdu);
     const T grp = -dot(grd, gro);
     const V3<T> grds = muls(mul(pscale, grd), grp);
-    const T t = (grp < T(0) ? T(-1) : T(1)) * tsqrt(dot(grds, grds));
-    if (t < t_near || t > t_far) return;
+    t = (grp < T(0) ? T(-1) : T(1)) * tsqrt(dot(grds, grds));
     const V3<T> gcrod = cross(grd, gro);
-    const T grayDist = dot(gcrod, gcrod);
-    if (grayDist < T(8)) hits.push_back({t, id});
+    grayDist = dot(gcrod, gcrod);
+    return !(t < t_near || t > t_far) && grayDist < T(8);
+}
+
+template <class T>
+inline void ray_prim(const V3<T>& o, const V3<T>& d, uint32_t id, const Gauss256& g, T t_near, T t_far,
+                     std::vector<HitRec<T>>& hits) {
+    T t, gd;
+    if (ray_gaussian<T>(o, d, g, t_near, t_far, t, gd)) hits.push_back({t, id});
 }
 
 // singlePassIntersectGaussian_sortNode_onlyShortStack, ELLIPSOID_BY_TRI
@@ -750,6 +786,39 @@ void oracle_ray_dirs(const void* cam, int W, int H, float* dirs) {
             p[2] = d.z;
         }
 }
+// Unit entry points of the intersection and SH restatements, for the
+// closed-form pins.  g: one 256-B record with inverted scales.
+int oracle_ray_gaussian_f32(const float* o, const float* d, const void* g, float t_near, float t_far, float* out) {
+    float t, gd;
+    const bool acc = ray_gaussian<float>({o[0], o[1], o[2]}, {d[0], d[1], d[2]}, *(const Gauss256*)g, t_near,
+                                         t_far, t, gd);
+    out[0] = t;
+    out[1] = gd;
+    return acc ? 1 : 0;
+}
+int oracle_ray_gaussian_f64(const double* o, const double* d, const void* g, double t_near, double t_far,
+                            double* out) {
+    double t, gd;
+    const bool acc = ray_gaussian<double>({o[0], o[1], o[2]}, {d[0], d[1], d[2]}, *(const Gauss256*)g, t_near,
+                                          t_far, t, gd);
+    out[0] = t;
+    out[1] = gd;
+    return acc ? 1 : 0;
+}
+void oracle_sh_basis_f32(const float* d, float* s) { sh_basis<float>({d[0], d[1], d[2]}, s); }
+void oracle_eval_sh_f32(const float* d, const void* g, float* rgb) {
+    const V3<float> c = eval_sh<float>({d[0], d[1], d[2]}, *(const Gauss256*)g);
+    rgb[0] = c.x;
+    rgb[1] = c.y;
+    rgb[2] = c.z;
+}
+void oracle_eval_sh_f64(const double* d, const void* g, double* rgb) {
+    const V3<double> c = eval_sh<double>({d[0], d[1], d[2]}, *(const Gauss256*)g);
+    rgb[0] = c.x;
+    rgb[1] = c.y;
+    rgb[2] = c.z;
+}
+
 int oracle_max_threads(void) { return omp_get_max_threads(); }
 
 } // extern "C"

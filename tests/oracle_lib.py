@@ -174,3 +174,36 @@ def render_hybrid_rays(nodes, offsets, g_inv, aabb, origins, dirs, spheres, max_
     fn(_p(nodes), _p(offsets), _p(g_inv), _p(aabb), _p(origins), _p(dirs), C.c_uint64(n), C.c_int(depth), C.c_int(1),
        C.c_int(len(sph)), _p(sph), None, None, C.c_int(max_bounces), _p(out), _p(seg), C.c_int(threads))
     return {"rgb": out, "segments": seg}
+
+
+def ray_gaussian(o, d, g_rec, t_near=1e-2, t_far=3.4e38, f64=False):
+    """rayPrimIntersect on one inverted-scale record: (accepted, t, grayDist)."""
+    g = np.ascontiguousarray(g_rec).reshape(-1)[:1]
+    if f64:
+        out = np.zeros(2, np.float64)
+        acc = lib().oracle_ray_gaussian_f64(_p(np.asarray(o, np.float64)), _p(np.asarray(d, np.float64)), _p(g),
+                                            C.c_double(t_near), C.c_double(t_far), _p(out))
+    else:
+        out = np.zeros(2, np.float32)
+        acc = lib().oracle_ray_gaussian_f32(_p(np.asarray(o, np.float32)), _p(np.asarray(d, np.float32)), _p(g),
+                                            C.c_float(t_near), C.c_float(t_far), _p(out))
+    return bool(acc), out[0], out[1]
+
+
+def sh_basis(d):
+    """The 16 SH basis scalars of eval_sh_final for direction d (float32)."""
+    s = np.zeros(16, np.float32)
+    lib().oracle_sh_basis_f32(_p(np.asarray(d, np.float32)), _p(s))
+    return s
+
+
+def eval_sh(d, g_rec, f64=False):
+    """eval_sh_final(3, d, g) of one record: rgb (3,)."""
+    g = np.ascontiguousarray(g_rec).reshape(-1)[:1]
+    if f64:
+        out = np.zeros(3, np.float64)
+        lib().oracle_eval_sh_f64(_p(np.asarray(d, np.float64)), _p(g), _p(out))
+    else:
+        out = np.zeros(3, np.float32)
+        lib().oracle_eval_sh_f32(_p(np.asarray(d, np.float32)), _p(g), _p(out))
+    return out
