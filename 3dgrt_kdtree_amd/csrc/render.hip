@@ -38,6 +38,7 @@ using namespace kdgrt_dev;
 namespace {
 
 constexpr int kXcds = 8;
+constexpr int kMaxDevices = 64;
 
 // Hit-buffer capacity of the plain view and ray-list launches (0.013 % of
 // the leaf visits of the 1M-Gaussian 1080p view have more hits and take a
@@ -320,6 +321,18 @@ struct RaysArgs {
 // uniform origins and directions: 313 -> 423 Mrays/s in this order).
 constexpr int kRayKeyBits = 30;
 constexpr uint64_t kRaySortMin = 65536;
+
+// Static scratch of kdgrt_render_rays_noalloc: two key buffers and two index
+// buffers of kNoAllocMaxRays entries, then the radix sort's temporaries
+// (36 MiB in all, held by every device the library runs on until the process
+// ends).  Host code never names the array directly: kdgrt_init resolves its
+// device address once per device (hipGetSymbolAddress) and the entry point
+// uses that address.
+constexpr uint64_t kNoAllocMaxRays = 1u << 21;
+constexpr size_t kNoAllocArr = (size_t)kNoAllocMaxRays * 4;
+constexpr size_t kNoAllocTemp = (size_t)4 << 20;
+constexpr size_t kNoAllocBytes = 4 * kNoAllocArr + kNoAllocTemp;
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
 
 __device__ __forceinline__ uint32_t spread3(uint32_t v) { // 9 bits -> every third bit
     uint32_t r = 0;
@@ -707,6 +720,7 @@ BatchKernel pick_batch_kernel(int k) {
 // ===========================================================================
 struct kdgrt_context {
     int device = 0;
+    void* noalloc_scratch = nullptr; // device address of `scratch` on this device
     hipDeviceProp_t prop{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::mutex mu;
@@ -930,8 +944,27 @@ kdgrt_status kdgrt_init(int device, kdgrt_context** out) {
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return set_error(KDGRT_ERR_NO_DEVICE, "init: no HIP device");
     if (device < 0 || device >= n) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "init: bad device index");
     HIP_TRY(hipSetDevice(device));
+    // The address of the static scratch array on this device, resolved once
+    // per device with the device current; every context on the device keeps it.
+    static std::mutex scratch_mu;
+    static void* scratch_ptr[kMaxDevices] = {};
+    if (device >= kMaxDevices) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "init: device index above 63");
+    void* sp = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(scratch_mu);
+        if (!scratch_ptr[device]) {
+            void* p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p)
+                return set_error(KDGRT_ERR_DEVICE, std::string("init: hipGetSymbolAddress(scratch): ") +
+                                                       hipGetErrorString(e));
+            scratch_ptr[device] = p;
+        }
+        sp = scratch_ptr[device];
+    }
     auto* c = new kdgrt_context();
     c->device = device;
+    c->noalloc_scratch = sp;
     if (hipGetDeviceProperties(&c->prop, device) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
         hipEventCreate(&c->ev1) != hipSuccess) {
         delete c;
@@ -1628,6 +1661,17 @@ kdgrt_status kdgrt_render_hybrid(kdgrt_context* ctx, const kdgrt_scene* sc, cons
     });
 }
 
+// Scene box of the ray keys: lower corner and 1 / extent per axis (0 for a
+// flat axis, whose cell index is then 0).
+static void ray_key_box(const DevScene& sc, float3& lo, float3& inv) {
+    const float ext[3] = {sc.bmax[0] - sc.bmin[0], sc.bmax[1] - sc.bmin[1], sc.bmax[2] - sc.bmin[2]};
+    lo = make_float3(sc.bmin[0], sc.bmin[1], sc.bmin[2]);
+    inv = make_float3(ext[0] > 0 ? 1.0f / ext[0] : 0.0f, ext[1] > 0 ? 1.0f / ext[1] : 0.0f,
+                      ext[2] > 0 ? 1.0f / ext[2] : 0.0f);
+}
+
+static uint32_t ray_key_grid(uint32_t n) { return (uint32_t)std::min<uint64_t>(((uint64_t)n + 255) / 256, 8192); }
+
 // Ray-key order of a ray list into perm (n entries) on stream s.  The
 // scratch (keys, indices, sort temporaries) comes from the stream-ordered
 // allocator and is released on s after the sort.
@@ -1637,24 +1681,19 @@ static kdgrt_status ray_order(const DevScene& sc, const float* ro, const float* 
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
                                                (const uint32_t*)nullptr, (uint32_t*)nullptr, n, 0, kRayKeyBits, s));
     const size_t arr = ((size_t)n * 4 + 255) & ~(size_t)255;
-    void* scratch = nullptr;
-    HIP_TRY(hipMallocAsync(&scratch, 3 * arr + temp, s));
-    char* base = (char*)scratch;
+    void* block = nullptr;
+    HIP_TRY(hipMallocAsync(&block, 3 * arr + temp, s));
+    char* base = (char*)block;
     uint32_t* k_in = (uint32_t*)base;
     uint32_t* k_out = (uint32_t*)(base + arr);
     uint32_t* v_in = (uint32_t*)(base + 2 * arr);
     void* tmp = base + 3 * arr;
-    const float* bmin = sc.bmin;
-    const float* bmax = sc.bmax;
-    const float ext[3] = {bmax[0] - bmin[0], bmax[1] - bmin[1], bmax[2] - bmin[2]};
-    const float3 lo = make_float3(bmin[0], bmin[1], bmin[2]);
-    const float3 inv = make_float3(ext[0] > 0 ? 1.0f / ext[0] : 0.0f, ext[1] > 0 ? 1.0f / ext[1] : 0.0f,
-                                   ext[2] > 0 ? 1.0f / ext[2] : 0.0f);
-    const uint32_t kgrid = (uint32_t)std::min<uint64_t>(((uint64_t)n + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_ray_keys, dim3(kgrid), dim3(256), 0, s, ro, rd, n, lo, inv, k_in, v_in);
+    float3 lo, inv;
+    ray_key_box(sc, lo, inv);
+    hipLaunchKernelGGL(k_ray_keys, dim3(ray_key_grid(n)), dim3(256), 0, s, ro, rd, n, lo, inv, k_in, v_in);
     HIP_TRY(hipGetLastError());
     const hipError_t e = hipcub::DeviceRadixSort::SortPairs(tmp, temp, k_in, k_out, v_in, perm, n, 0, kRayKeyBits, s);
-    HIP_TRY(hipFreeAsync(scratch, s));
+    HIP_TRY(hipFreeAsync(block, s));
     HIP_TRY(e);
     return KDGRT_OK;
 }
@@ -1673,20 +1712,66 @@ static kdgrt_status launch_rays(kdgrt_context* ctx, const kdgrt_scene* sc, const
     // Lists of at least kRaySortMin rays are traced in ray-key order (the
     // sort costs less than the coherence it buys); the scratch comes from the
     // stream-ordered allocator, so the call stays asynchronous.
-    void* scratch = nullptr;
+    void* perm = nullptr;
     if (n >= kRaySortMin && n <= 0x7FFFFFFFull) {
-        HIP_TRY(hipMallocAsync(&scratch, 4 * n, s));
-        const kdgrt_status st = ray_order(sc->dev, ro, rd, (uint32_t)n, (uint32_t*)scratch, s);
+        HIP_TRY(hipMallocAsync(&perm, 4 * n, s));
+        const kdgrt_status st = ray_order(sc->dev, ro, rd, (uint32_t)n, (uint32_t*)perm, s);
         if (st != KDGRT_OK) {
-            hipFreeAsync(scratch, s);
+            hipFreeAsync(perm, s);
             return st;
         }
-        a.perm = (const uint32_t*)scratch;
+        a.perm = (const uint32_t*)perm;
     }
     if (dbg) hipLaunchKernelGGL((k_rays<kViewK, true>), grid, block, lds, s, a);
     else hipLaunchKernelGGL((k_rays<kViewK, false>), grid, block, lds, s, a);
     HIP_TRY(hipGetLastError());
-    if (scratch) HIP_TRY(hipFreeAsync(scratch, s));
+    if (perm) HIP_TRY(hipFreeAsync(perm, s));
+    return KDGRT_OK;
+}
+
+// Allocation-free ray lists: the ray-key sort of kdgrt_render_rays works in
+// the file-scope device array `scratch` (resolved per device at init)
+// instead of stream-ordered allocations.  Layout: keys (two buffers), ray
+// indices (two buffers, the sort's ping-pong pair), then the radix sort's
+// temporaries.  Every call on one device shares the array, so calls that
+// overlap on one device race; nothing in it has to start at zero (the key
+// kernel writes every key and index, the sort every output).
+kdgrt_status kdgrt_render_rays_noalloc(kdgrt_context* ctx, const kdgrt_scene* sc, const float* ro, const float* rd,
+                                       uint64_t n, int depth, float* out, void* stream) {
+    if (!ctx || !sc || (n && (!ro || !rd || !out)))
+        return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render_rays_noalloc: null");
+    if (depth < 1 || depth > 32)
+        return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render_rays_noalloc: stack depth must be 1..32");
+    if (n > kNoAllocMaxRays)
+        return set_error(KDGRT_ERR_CAPACITY, "render_rays_noalloc: " + std::to_string(n) +
+                                                 " rays exceed the static scratch's " +
+                                                 std::to_string(kNoAllocMaxRays) + "; use kdgrt_render_rays");
+    if (n == 0) return KDGRT_OK;
+    if (!ctx->noalloc_scratch) return set_error(KDGRT_ERR_DEVICE, "render_rays_noalloc: scratch not resolved");
+    RaysArgs a{sc->dev, ro, rd, n, depth, (float4*)out, nullptr, nullptr, nullptr};
+    hipStream_t s = (hipStream_t)stream;
+    if (n >= kRaySortMin) {
+        char* base = (char*)ctx->noalloc_scratch;
+        hipcub::DoubleBuffer<uint32_t> keys((uint32_t*)base, (uint32_t*)(base + kNoAllocArr));
+        hipcub::DoubleBuffer<uint32_t> idx((uint32_t*)(base + 2 * kNoAllocArr), (uint32_t*)(base + 3 * kNoAllocArr));
+        void* tmp = base + 4 * kNoAllocArr;
+        size_t temp = 0;
+        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, keys, idx, (uint32_t)n, 0, kRayKeyBits, s));
+        if (temp > kNoAllocTemp)
+            return set_error(KDGRT_ERR_CAPACITY, "render_rays_noalloc: the radix sort needs " + std::to_string(temp) +
+                                                     " B of temporaries, the static scratch holds " +
+                                                     std::to_string(kNoAllocTemp));
+        float3 lo, inv;
+        ray_key_box(sc->dev, lo, inv);
+        hipLaunchKernelGGL(k_ray_keys, dim3(ray_key_grid((uint32_t)n)), dim3(256), 0, s, ro, rd, (uint32_t)n, lo, inv,
+                           keys.Current(), idx.Current());
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, temp, keys, idx, (uint32_t)n, 0, kRayKeyBits, s));
+        a.perm = idx.Current();
+    }
+    hipLaunchKernelGGL((k_rays<kViewK, false>), dim3(interleave_grid<64>((n + 63) / 64)), dim3(64),
+                       stack_lds_bytes(depth, 64), s, a);
+    HIP_TRY(hipGetLastError());
     return KDGRT_OK;
 }
 

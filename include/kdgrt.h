@@ -251,6 +251,18 @@ KDGRT_API kdgrt_status kdgrt_render_rays_debug(kdgrt_context* ctx, const kdgrt_s
                                                const float* origins, const float* dirs, uint64_t n_rays,
                                                int stack_depth, float* out_rgbt, int32_t* counters,
                                                uint64_t* id_hash, void* stream);
+/* No counterpart in the reference: kdgrt_render_rays for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same kernels and results as
+ * kdgrt_render_rays (bit for bit); the ray-key sort works in a static device
+ * array of 36 MiB per device, resolved at kdgrt_init, and the call makes no
+ * hipMalloc/hipFree or other allocating call.  At most 2,097,152 rays per
+ * call (KDGRT_ERR_CAPACITY above that).  Calls share the device's array:
+ * calls that overlap on one device race, so serialise them (one stream, or
+ * wait between streams). */
+KDGRT_API kdgrt_status kdgrt_render_rays_noalloc(kdgrt_context* ctx, const kdgrt_scene* scene,
+                                                 const float* origins, const float* dirs, uint64_t n_rays,
+                                                 int stack_depth, float* out_rgbt, void* stream);
 KDGRT_API kdgrt_status kdgrt_warm_up(kdgrt_context* ctx, const kdgrt_scene* scene, void* stream);
 
 /* ---------------- hybrid render: Gaussians + reflective/refractive secondary primitives ---------------- */
