@@ -1579,12 +1579,21 @@ kdgrt_status render_served(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgr
     return render_plain(ctx, sc, cam, W, H, depth, KDGRT_OUT_RGBT_F32, out, stream, kSchedNoAlloc);
 }
 
+// Logical tiles of the schedule buffers that views up to max_w x max_h need.
+static uint32_t schedule_tiles(int max_w, int max_h) {
+    const uint64_t n = interleave_grid<64>((uint64_t)((max_w + 63) / 64) * (uint64_t)((max_h + 63) / 64) * 64);
+    return (uint32_t)std::min<uint64_t>(n, kSchedMaxTiles);
+}
+
+uint64_t tile_schedule_bytes(int max_w, int max_h) {
+    return (3ull * schedule_tiles(max_w, max_h) + kDensCells) * sizeof(uint32_t);
+}
+
 kdgrt_status reserve_tile_schedule(kdgrt_scene* sc, int max_w, int max_h) {
     if (!sc || max_w <= 0 || max_h <= 0) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "reserve_tile_schedule");
     hipSetDevice(sc->ctx->device);
-    const uint64_t n = interleave_grid<64>((uint64_t)((max_w + 63) / 64) * (uint64_t)((max_h + 63) / 64) * 64);
     std::lock_guard<std::mutex> lock(sc->cone_mu);
-    return sched_grow(sc, (uint32_t)std::min<uint64_t>(n, kSchedMaxTiles));
+    return sched_grow(sc, schedule_tiles(max_w, max_h));
 }
 } // namespace kdgrt
 } // extern "C++"

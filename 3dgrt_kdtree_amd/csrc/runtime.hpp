@@ -20,6 +20,8 @@ kdgrt_status render_served(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgr
                            int depth, void* out, hipStream_t stream);
 // Tile-schedule buffers of a scene for views up to max_w x max_h pixels.
 kdgrt_status reserve_tile_schedule(kdgrt_scene* sc, int max_w, int max_h);
+// Device bytes that reserve_tile_schedule(sc, max_w, max_h) allocates for a scene.
+uint64_t tile_schedule_bytes(int max_w, int max_h);
 int context_device(const kdgrt_context* ctx);
 
 } // namespace kdgrt

@@ -59,6 +59,7 @@ struct Req {
     double arrival = 0, admitted = 0, launched = 0, done = 0;
     uint64_t bytes = 0, off = 0;
     bool has_mem = false;
+    bool collecting = false; // a kdgrt_server_wait is copying the result out
     uint32_t batch_id = 0, batch_views = 0;
 };
 
@@ -146,6 +147,7 @@ struct kdgrt_server {
     std::vector<bool> slot_busy;
     std::vector<Flight> flights;
     hipStream_t copy_stream = nullptr;
+    std::vector<hipStream_t> copy_pool; // result-copy streams of kdgrt_server_wait (copy_stream among them)
     double sim_gpu_free = 0;
 
     std::thread worker;
@@ -346,8 +348,9 @@ kdgrt_status kdgrt_server_add_scene(kdgrt_server* s, const kdgrt_tree* tree, con
     if (!s || !tree || !scene_id) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "server_add_scene: null");
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->started) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "server_add_scene: server already started");
-    const uint64_t bytes =
-        kdgrt_scene_bytes_estimate((uint32_t)tree->nodes.size(), tree->offsets.size(), n_g);
+    // the scene's arrays plus, on a device, the tile-schedule buffers reserved for it below
+    const uint64_t bytes = kdgrt_scene_bytes_estimate((uint32_t)tree->nodes.size(), tree->offsets.size(), n_g) +
+                           (s->sim ? 0 : kdgrt::tile_schedule_bytes(kServeMaxEdge, kServeMaxEdge));
     if (s->scene_bytes + bytes > s->cfg.hbm_budget_bytes)
         return set_error(KDGRT_ERR_CAPACITY, "server_add_scene: scene does not fit the HBM budget");
     kdgrt_scene* sc = nullptr;
@@ -381,6 +384,7 @@ kdgrt_status kdgrt_server_add_resident_scene(kdgrt_server* s, kdgrt_scene* scene
     uint64_t bytes = 0;
     kdgrt_status st = kdgrt_scene_device_bytes(scene, &bytes);
     if (st != KDGRT_OK) return st;
+    bytes += kdgrt::tile_schedule_bytes(kServeMaxEdge, kServeMaxEdge); // reserved for the scene below
     if (s->scene_bytes + bytes > s->cfg.hbm_budget_bytes)
         return set_error(KDGRT_ERR_CAPACITY, "server_add_resident_scene: scene does not fit the HBM budget");
     st = kdgrt::reserve_tile_schedule(scene, kServeMaxEdge, kServeMaxEdge);
@@ -423,6 +427,7 @@ kdgrt_status kdgrt_server_start(kdgrt_server* s) {
         if (e != hipSuccess) return set_error(KDGRT_ERR_OUT_OF_MEMORY, "server_start: output arena allocation failed");
         if (hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess)
             return set_error(KDGRT_ERR_DEVICE, "server_start: stream");
+        s->copy_pool.push_back(s->copy_stream);
         for (int i = 0; i < n; ++i) {
             hipStream_t st;
             hipEvent_t ev;
@@ -477,6 +482,7 @@ kdgrt_status kdgrt_server_wait(kdgrt_server* s, uint64_t id, double timeout_ms, 
     auto it = s->reqs.find(id);
     if (it == s->reqs.end()) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "server_wait: unknown request");
     Req* r = it->second;
+    if (r->collecting) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "server_wait: request already being collected");
     auto finished = [&] { return r->state >= kDone; };
     if (timeout_ms < 0) s->cv_done.wait(lk, finished);
     else if (!s->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), finished))
@@ -487,10 +493,30 @@ kdgrt_status kdgrt_server_wait(kdgrt_server* s, uint64_t id, double timeout_ms, 
         if (s->sim) {
             std::memset(host_rgbt, 0, r->bytes);
         } else {
+            // The copy runs without the lock, so the scheduler thread keeps
+            // retiring and launching batches meanwhile.  The request stays in
+            // s->reqs and its arena region stays allocated until it is erased
+            // below (only this call erases a finished request), and each
+            // waiter copies on its own stream.
+            // Concurrent waiters of other requests take copy streams from a
+            // pool (created on demand, destroyed with the server).
             const void* src = (const char*)s->d_arena + r->off;
-            if (hipMemcpyAsync(host_rgbt, src, r->bytes, hipMemcpyDeviceToHost, s->copy_stream) != hipSuccess ||
-                hipStreamSynchronize(s->copy_stream) != hipSuccess)
-                st = set_error(KDGRT_ERR_DEVICE, "server_wait: result copy failed");
+            const size_t nbytes = r->bytes;
+            r->collecting = true;
+            hipStream_t cs = nullptr;
+            if (!s->copy_pool.empty()) {
+                cs = s->copy_pool.back();
+                s->copy_pool.pop_back();
+            } else if (hipSetDevice(kdgrt::context_device(s->ctx)) != hipSuccess ||
+                       hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
+                cs = nullptr;
+            }
+            lk.unlock();
+            const bool ok = cs && hipMemcpyAsync(host_rgbt, src, nbytes, hipMemcpyDeviceToHost, cs) == hipSuccess &&
+                            hipStreamSynchronize(cs) == hipSuccess;
+            lk.lock();
+            if (cs) s->copy_pool.push_back(cs);
+            if (!ok) st = set_error(KDGRT_ERR_DEVICE, "server_wait: result copy failed");
         }
     }
     if (info) {
@@ -563,7 +589,7 @@ kdgrt_status kdgrt_server_destroy(kdgrt_server* s) {
         for (auto sl : s->slots) kdgrt::slot_destroy(sl);
         for (auto e : s->events) hipEventDestroy(e);
         for (auto st : s->streams) hipStreamDestroy(st);
-        if (s->copy_stream) hipStreamDestroy(s->copy_stream);
+        for (auto cs : s->copy_pool) hipStreamDestroy(cs);
         if (s->d_arena) hipFree(s->d_arena);
         for (auto& kv : s->scenes)
             if (kv.second.first && s->scene_owned[kv.first]) kdgrt_scene_destroy(kv.second.first);
