@@ -500,10 +500,99 @@ class Device:
             b.free()
         return res
 
+    def _touch_maps(self, scene):
+        sizes = {"nodes": scene.n_nodes, "quads": scene.n_nodes, "slots_pretested": scene.n_offsets,
+                 "slots_tested": scene.n_offsets, "gaussians_evaluated": scene.n_gaussians,
+                 "gaussians_blended": scene.n_gaussians}
+        bufs = {k: self.alloc(max(1, n)) for k, n in sizes.items()}
+        for k, b in bufs.items():
+            check(lib().kdgrt_memset(self.h, b.ptr, C.c_int(0), C.c_size_t(max(1, sizes[k]))))
+        m = TouchMaps(*[bufs[k].ptr for k, _ in TouchMaps._fields_])
+        return m, bufs, sizes
+
+    def _touch_result(self, bufs, sizes):
+        res = {k: b.download(np.uint8, (sizes[k],)).astype(bool) for k, b in bufs.items()}
+        for b in bufs.values():
+            b.free()
+        return res
+
+    def valu_probe(self, iters=1 << 14, stream=None, reps=3):
+        """The VALU-issue ceiling (k_valu_probe): fused multiply-adds issued per
+        second over `reps` timed launches after one warm-up."""
+        waves, per = C.c_uint64(), C.c_uint64()
+        check(lib().kdgrt_valu_probe(self.h, C.c_int(iters), stream, C.byref(waves), C.byref(per)))
+        self.sync(stream)
+        self.timer_begin(stream)
+        for _ in range(reps):
+            check(lib().kdgrt_valu_probe(self.h, C.c_int(iters), stream, C.byref(waves), C.byref(per)))
+        ms = self.timer_end(stream) / reps
+        self.sync(stream)
+        return {"kernel": "k_valu_probe", "waves": waves.value, "fma_per_wave": per.value, "ms": ms,
+                "valu_instr_per_s": waves.value * per.value / (ms * 1e-3)}
+
+    def render_touch(self, scene, ci, w, h, depth=12):
+        """Touch maps of one plain view (kdgrt_render_touch): which node,
+        quad, leaf-slot and Gaussian records the launch reads."""
+        out = self.alloc(w * h * 16)
+        m, bufs, sizes = self._touch_maps(scene)
+        check(lib().kdgrt_render_touch(self.h, scene.h, C.byref(ci), C.c_int(w), C.c_int(h), C.c_int(depth), out.ptr,
+                                       C.byref(m), None))
+        self.sync()
+        res = self._touch_result(bufs, sizes)
+        res["rgbt"] = out.download(np.float32, (h, w, 4))
+        out.free()
+        return res
+
+    def render_rays_touch(self, scene, origins, dirs, depth=12):
+        """Touch maps of a ray list (kdgrt_render_rays_touch)."""
+        origins = np.ascontiguousarray(origins, np.float32)
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        n = len(origins)
+        o, d, out = self.upload(origins), self.upload(dirs), self.alloc(n * 16)
+        m, bufs, sizes = self._touch_maps(scene)
+        check(lib().kdgrt_render_rays_touch(self.h, scene.h, o.ptr, d.ptr, C.c_uint64(n), C.c_int(depth), out.ptr,
+                                            C.byref(m), None))
+        self.sync()
+        res = self._touch_result(bufs, sizes)
+        res["rgbt"] = out.download(np.float32, (n, 4))
+        for b in (o, d, out):
+            b.free()
+        return res
+
     def close(self):
         if self.h:
             lib().kdgrt_shutdown(self.h)
             self.h = None
+
+
+class TouchMaps(C.Structure):
+    _fields_ = [("nodes", C.c_void_p), ("quads", C.c_void_p), ("slots_pretested", C.c_void_p),
+                ("slots_tested", C.c_void_p), ("gaussians_evaluated", C.c_void_p),
+                ("gaussians_blended", C.c_void_p)]
+
+
+def compulsory_bytes(touch, offsets, n_rays, explicit_rays=False):
+    """Compulsory bytes of a launch: every record its rays read, counted once.
+
+    reference layout (cudaRenderer.cu:292-307, 1079-1085, 1595): 8-B node
+    records visited, 4-B leaf-list entries, the 48-B geometry of each Gaussian
+    tested, the rest of the 256-B record of each Gaussian evaluated, 16 B of
+    output per ray (+24 B of ray input for explicit rays).
+    this layout: the root node, 48-B child-pair quads loaded, a 16-B pre-test
+    record per slot pre-tested (cone records for views, bound spheres for ray
+    lists), the 80-B geometry record of each slot exact-tested, the 192-B SH
+    record of each Gaussian blended, the same output and input."""
+    offsets = np.asarray(offsets)
+    io = n_rays * (16 + (24 if explicit_rays else 0))
+    tested = np.unique(offsets[np.flatnonzero(touch["slots_pretested"])])
+    evaluated = int(touch["gaussians_evaluated"].sum())
+    ref = {"nodes": 8 * int(touch["nodes"].sum()), "leaf_lists": 4 * int(touch["slots_pretested"].sum()),
+           "gaussian_geometry": 48 * len(tested), "gaussian_rest": (256 - 48) * evaluated, "io": io}
+    ours = {"root": 8, "quads": 48 * int(touch["quads"].sum()), "pretest": 16 * int(touch["slots_pretested"].sum()),
+            "geometry": 80 * int(touch["slots_tested"].sum()), "sh": 192 * int(touch["gaussians_blended"].sum()),
+            "io": io}
+    return {"reference_layout": sum(ref.values()), "this_layout": sum(ours.values()),
+            "reference_layout_parts": ref, "this_layout_parts": ours}
 
 
 class DeviceBuffer:
@@ -536,6 +625,8 @@ class Scene:
         g = np.ascontiguousarray(g, dtype=GAUSSIAN_DTYPE)
         check(lib().kdgrt_scene_create_from_tree(dev.h, tree.h, _p(g), C.c_uint64(len(g)),
                                                  C.c_int(1 if scales_inverted else 0), C.byref(self.h)))
+        st = tree.stats()
+        self.n_nodes, self.n_offsets, self.n_gaussians = int(st["n_nodes"]), int(st["n_offsets"]), len(g)
 
     def device_bytes(self):
         b = C.c_uint64()

@@ -63,6 +63,7 @@ struct ViewArgs {
     uint64_t* hashes;
     uint32_t* seq;
     int seq_cap;
+    uint8_t* touch[6]; // DBG: touch maps (node, quad, pre-test, exact test, evaluated, blended), each may be null
 };
 
 // Hardware dispatches workgroup b to XCD b % 8.  Logical blocks are taken
@@ -112,6 +113,12 @@ __global__ __launch_bounds__(64) void k_view(ViewArgs a) {
         dbg.seq = a.seq ? a.seq + rid * (size_t)a.seq_cap : nullptr;
         dbg.seq_cap = a.seq_cap;
         dbg.nseq = 0;
+        dbg.t_node = a.touch[0];
+        dbg.t_quad = a.touch[1];
+        dbg.t_pre = a.touch[2];
+        dbg.t_exact = a.touch[3];
+        dbg.t_eval = a.touch[4];
+        dbg.t_blend = a.touch[5];
     }
     if (PROF) {
         for (int i = 0; i < 5; ++i) dbg.cyc[i] = 0;
@@ -312,6 +319,7 @@ struct RaysArgs {
     int32_t* counters;
     uint64_t* hashes;
     const uint32_t* perm; // processing order (ray indices), or null for list order
+    uint8_t* touch[6];    // DBG: touch maps as in ViewArgs
 };
 
 // Sort key of an explicit ray: its direction octant (3 bits), then the Morton
@@ -374,6 +382,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DBG ? 1 : 4)
         dbg.seq = nullptr;
         dbg.seq_cap = 0;
         dbg.nseq = 0;
+        dbg.t_node = a.touch[0];
+        dbg.t_quad = a.touch[1];
+        dbg.t_pre = a.touch[2];
+        dbg.t_exact = a.touch[3];
+        dbg.t_eval = a.touch[4];
+        dbg.t_blend = a.touch[5];
     }
     extern __shared__ uint2 lds_stack[];
     trace_ray<K, DBG>(a.sc, o, d, a.depth, lds_stack, (int)threadIdx.x, rgb, A, &dbg);
@@ -515,6 +529,26 @@ __global__ __launch_bounds__(256) void k_to_u8(const float* __restrict__ fb, int
 
 __global__ __launch_bounds__(256) void k_copy(float4* __restrict__ dst, const float4* __restrict__ src, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) dst[i] = src[i];
+}
+
+// VALU-issue ceiling: eight independent fused multiply-add chains per lane,
+// kValuChains * iters v_fma_f32 per wavefront and nothing else in the loop
+// but scalar loop control, at full occupancy.  Its instruction rate
+// (kdgrt_valu_probe) is the measured peak that the trace kernels' VALU
+// instruction counts are compared with.
+constexpr int kValuChains = 8;
+__global__ __launch_bounds__(256) void k_valu_probe(float* __restrict__ out, int iters, float b, float c) {
+    float a[kValuChains];
+#pragma unroll
+    for (int k = 0; k < kValuChains; ++k) a[k] = (float)(threadIdx.x + k);
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int k = 0; k < kValuChains; ++k) a[k] = __builtin_fmaf(a[k], b, c);
+    }
+    float r = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kValuChains; ++k) r += a[k];
+    out[blockIdx.x * 256 + threadIdx.x] = r;
 }
 
 // Per-view cone records for the pre-test of rays leaving one origin e
@@ -703,8 +737,8 @@ typedef void (*BatchKernel)(BatchArgs);
 
 // Dynamic LDS of a launch: per wavefront, one 8-B short-stack ring entry per
 // lane and level plus each lane's 64-B SH basis.
-static inline size_t stack_lds_bytes(int depth, int block_threads) {
-    return wave_lds_bytes(depth) * (size_t)(block_threads / 64);
+static inline size_t stack_lds_bytes(int depth, int block_threads, bool dbg = false) {
+    return wave_lds_bytes(depth, dbg) * (size_t)(block_threads / 64);
 }
 
 BatchKernel pick_batch_kernel(int k) {
@@ -1080,6 +1114,20 @@ kdgrt_status kdgrt_copy_f32x4(kdgrt_context* ctx, void* dst, const void* src, ui
     return KDGRT_OK;
 }
 
+kdgrt_status kdgrt_valu_probe(kdgrt_context* ctx, int iters, void* stream, uint64_t* waves, uint64_t* fma_per_wave) {
+    if (!ctx || iters < 1 || iters > (1 << 24)) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "valu_probe: iters 1..2^24");
+    // eight waves per SIMD on every CU: blocks of four waves, two per SIMD each
+    const int blocks = std::max(1, ctx->prop.multiProcessorCount) * 8;
+    float* out = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&out, (size_t)blocks * 256 * sizeof(float), (hipStream_t)stream));
+    hipLaunchKernelGGL(k_valu_probe, dim3(blocks), dim3(256), 0, (hipStream_t)stream, out, iters, 0.999f, 1e-3f);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipFreeAsync(out, (hipStream_t)stream));
+    if (waves) *waves = (uint64_t)blocks * 4;
+    if (fma_per_wave) *fma_per_wave = (uint64_t)kValuChains * (uint64_t)iters;
+    return KDGRT_OK;
+}
+
 kdgrt_status kdgrt_framebuffer_to_u8(kdgrt_context* ctx, const float* fb, int width, int height, int in_channels,
                                      uint8_t* out, int out_channels, int flip_rows, void* stream) {
     if (!ctx || !fb || !out) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "framebuffer_to_u8: null");
@@ -1321,6 +1369,7 @@ static void fill_view_args(ViewArgs& a, const kdgrt_scene* sc, const kdgrt_camer
     a.hashes = nullptr;
     a.seq = nullptr;
     a.seq_cap = 0;
+    for (int i = 0; i < 6; ++i) a.touch[i] = nullptr;
 }
 
 // Logical tiles of a view launch (the grid of k_view and k_hybrid).
@@ -1545,10 +1594,10 @@ typedef void (*ViewKernel)(ViewArgs);
 enum SchedUse { kSchedNone, kSchedFull, kSchedNoAlloc };
 
 static kdgrt_status launch_view(const kdgrt_scene* csc, ViewArgs& a, ViewKernel kern, hipStream_t s,
-                                SchedUse sched = kSchedNone) {
+                                SchedUse sched = kSchedNone, bool dbg = false) {
     return with_cone_table(csc, a, s, sched == kSchedNone ? -1 : 0, sched == kSchedFull,
                            [&](kdgrt_scene*) -> kdgrt_status {
-        hipLaunchKernelGGL(kern, dim3(view_tiles(a)), dim3(64), stack_lds_bytes(a.depth, 64), s, a);
+        hipLaunchKernelGGL(kern, dim3(view_tiles(a)), dim3(64), stack_lds_bytes(a.depth, 64, dbg), s, a);
         return KDGRT_OK;
     });
 }
@@ -1610,7 +1659,27 @@ kdgrt_status kdgrt_render_debug(kdgrt_context* ctx, const kdgrt_scene* sc, const
     a.hashes = hashes;
     a.seq = seq;
     a.seq_cap = seq ? seq_cap : 0;
-    return launch_view(sc, a, k_view<kViewK, true, KDGRT_OUT_RGBT_F32>, (hipStream_t)stream);
+    return launch_view(sc, a, k_view<kViewK, true, KDGRT_OUT_RGBT_F32>, (hipStream_t)stream, kSchedNone, true);
+}
+
+static void touch_ptrs(const kdgrt_touch_maps* m, uint8_t* t[6]) {
+    t[0] = m->nodes;
+    t[1] = m->quads;
+    t[2] = m->slots_pretested;
+    t[3] = m->slots_tested;
+    t[4] = m->gaussians_evaluated;
+    t[5] = m->gaussians_blended;
+}
+
+kdgrt_status kdgrt_render_touch(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgrt_camera_info* cam, int W,
+                                int H, int depth, float* out, const kdgrt_touch_maps* maps, void* stream) {
+    kdgrt_status st = check_view(ctx, sc, cam, W, H, depth, out);
+    if (st != KDGRT_OK) return st;
+    if (!maps) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render_touch: null maps");
+    ViewArgs a;
+    fill_view_args(a, sc, cam, W, H, depth, out);
+    touch_ptrs(maps, a.touch);
+    return launch_view(sc, a, k_view<kViewK, true, KDGRT_OUT_RGBT_F32>, (hipStream_t)stream, kSchedNone, true);
 }
 
 kdgrt_status kdgrt_render_profile(kdgrt_context* ctx, const kdgrt_scene* sc, const kdgrt_camera_info* cam, int W,
@@ -1709,15 +1778,16 @@ static kdgrt_status ray_order(const DevScene& sc, const float* ro, const float* 
 
 static kdgrt_status launch_rays(kdgrt_context* ctx, const kdgrt_scene* sc, const float* ro, const float* rd,
                                 uint64_t n, int depth, float* out, int32_t* counters, uint64_t* hashes, bool dbg,
-                                void* stream) {
+                                void* stream, const kdgrt_touch_maps* maps = nullptr) {
     if (!ctx || !sc || (n && (!ro || !rd || !out))) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render_rays: null");
     if (depth < 1 || depth > 32) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render_rays: stack depth must be 1..32");
     if (n == 0) return KDGRT_OK;
     if (n > (uint64_t)0x7FFFF000 * 64ull) return set_error(KDGRT_ERR_CAPACITY, "render_rays: too many rays");
-    RaysArgs a{sc->dev, ro, rd, n, depth, (float4*)out, counters, hashes, nullptr};
+    RaysArgs a{sc->dev, ro, rd, n, depth, (float4*)out, counters, hashes, nullptr, {}};
+    if (maps) touch_ptrs(maps, a.touch);
     const dim3 grid(interleave_grid<64>((n + 63) / 64)), block(64);
     hipStream_t s = (hipStream_t)stream;
-    const size_t lds = stack_lds_bytes(depth, 64);
+    const size_t lds = stack_lds_bytes(depth, 64, dbg);
     // Lists of at least kRaySortMin rays are traced in ray-key order (the
     // sort costs less than the coherence it buys); the scratch comes from the
     // stream-ordered allocator, so the call stays asynchronous.
@@ -1757,7 +1827,7 @@ kdgrt_status kdgrt_render_rays_noalloc(kdgrt_context* ctx, const kdgrt_scene* sc
                                                  std::to_string(kNoAllocMaxRays) + "; use kdgrt_render_rays");
     if (n == 0) return KDGRT_OK;
     if (!ctx->noalloc_scratch) return set_error(KDGRT_ERR_DEVICE, "render_rays_noalloc: scratch not resolved");
-    RaysArgs a{sc->dev, ro, rd, n, depth, (float4*)out, nullptr, nullptr, nullptr};
+    RaysArgs a{sc->dev, ro, rd, n, depth, (float4*)out, nullptr, nullptr, nullptr, {}};
     hipStream_t s = (hipStream_t)stream;
     if (n >= kRaySortMin) {
         char* base = (char*)ctx->noalloc_scratch;
@@ -1793,6 +1863,12 @@ kdgrt_status kdgrt_render_rays_debug(kdgrt_context* ctx, const kdgrt_scene* sc, 
                                      uint64_t n, int depth, float* out, int32_t* counters, uint64_t* hashes,
                                      void* stream) {
     return launch_rays(ctx, sc, ro, rd, n, depth, out, counters, hashes, true, stream);
+}
+
+kdgrt_status kdgrt_render_rays_touch(kdgrt_context* ctx, const kdgrt_scene* sc, const float* ro, const float* rd,
+                                     uint64_t n, int depth, float* out, const kdgrt_touch_maps* maps, void* stream) {
+    if (!maps) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "render_rays_touch: null maps");
+    return launch_rays(ctx, sc, ro, rd, n, depth, out, nullptr, nullptr, true, stream, maps);
 }
 
 kdgrt_status kdgrt_warm_up(kdgrt_context* ctx, const kdgrt_scene* sc, void* stream) {

@@ -78,6 +78,18 @@ struct RayDbg {
     uint64_t cyc[5];
     uint64_t tlast;
     uint32_t pretests, survivors; // PROF variant: candidates pre-tested / passed on to the exact test
+    // Touch maps (DBG variant, each may be null): a byte set to 1 for every
+    // record the ray reads -- node records visited (by node index), child-pair
+    // quad records loaded (by first-child index), leaf slots pre-tested and
+    // exact-tested (by slot), Gaussians entering the composite loop and
+    // Gaussians blended (by ID).  Lanes store the same value, so the races
+    // are benign.
+    uint8_t* t_node;
+    uint8_t* t_quad;
+    uint8_t* t_pre;
+    uint8_t* t_exact;
+    uint8_t* t_eval;
+    uint8_t* t_blend;
 };
 
 #define KD_PROF_MARK(ph)                                                                                        \
@@ -327,6 +339,7 @@ struct RayState {
     float t_near, t_far, t_scene_far;
     uint2 node;
     uint32_t top, quant; // ring stack index and fill (cudaRenderer.cu:189-210)
+    uint32_t nidx;       // index of st.node (DBG variant only: touch maps)
     float rgb[3];
     float A;
     int guard;
@@ -367,6 +380,7 @@ __device__ __forceinline__ bool ray_begin(RayState& st, const DevScene& sc, cons
     st.top = (uint32_t)depth - 1u;
     st.quant = 0;
     st.guard = 0;
+    st.nidx = 0;
     if (!((t_far >= t_near) & (t_far >= 0.f))) return false;
     st.node = sc.nodes[0];
     return true;
@@ -381,8 +395,10 @@ __device__ __forceinline__ bool ray_begin(RayState& st, const DevScene& sc, cons
 // pop needs no global load.
 // One inner-node step (cudaRenderer.cu:1555-1573) with the node's children
 // n0, n1 at hand; returns which child is next (0 near-side n0, 1 n1).
+template <bool DBG>
 __device__ __forceinline__ uint32_t node_step(RayState& st, const uint2 n0, const uint2 n1, int depth,
-                                              uint32_t* __restrict__ stk) {
+                                              uint32_t* __restrict__ stk, RayDbg* dbg) {
+    if (DBG && dbg->t_node) dbg->t_node[st.nidx] = 1;
     const uint32_t axis = st.node.x & 3u;
     const float oa = sel_axis(axis, st.ox, st.oy, st.oz);
     const float ia = sel_axis(axis, st.ix, st.iy, st.iz);
@@ -396,8 +412,11 @@ __device__ __forceinline__ uint32_t node_step(RayState& st, const uint2 n0, cons
         stk[64 * st.top] = far.x;
         stk[64 * (depth + st.top)] = far.y;
         stk[64 * (2 * depth + st.top)] = __float_as_uint(st.t_far);
+        // DBG: a fourth plane holds the pushed node's index (child + (sign ^ 1))
+        if (DBG) stk[64 * (3 * depth + st.top)] = (st.node.x >> 3) + (sign ^ 1u);
         st.t_far = t_split;
     }
+    if (DBG) st.nidx = (st.node.x >> 3) + pick;
     st.node = pick ? n1 : n0;
     return pick;
 }
@@ -411,14 +430,16 @@ __device__ __forceinline__ void descend(RayState& st, const DevScene& sc, int de
                                         RayDbg* dbg) {
     while ((st.node.x & 7u) != 3u) {
         const uint4* qp = sc.quads + 3 * (size_t)(st.node.x >> 3);
+        if (DBG && dbg->t_quad) dbg->t_quad[st.node.x >> 3] = 1;
         const uint4 q0 = qp[0], q1 = qp[1], q2 = qp[2];
-        const uint32_t pick = node_step(st, make_uint2(q0.x, q0.y), make_uint2(q0.z, q0.w), depth, stk);
+        const uint32_t pick = node_step<DBG>(st, make_uint2(q0.x, q0.y), make_uint2(q0.z, q0.w), depth, stk, dbg);
         if (DBG) dbg->cnt[0]++;
         if ((st.node.x & 7u) == 3u) break;
         const uint4 g = pick ? q2 : q1;
-        node_step(st, make_uint2(g.x, g.y), make_uint2(g.z, g.w), depth, stk);
+        node_step<DBG>(st, make_uint2(g.x, g.y), make_uint2(g.z, g.w), depth, stk, dbg);
         if (DBG) dbg->cnt[0]++;
     }
+    if (DBG && dbg->t_node) dbg->t_node[st.nidx] = 1; // the leaf
 }
 
 // After a leaf (cudaRenderer.cu:1690-1708): termination test, then pop or
@@ -431,9 +452,13 @@ __device__ __forceinline__ bool next_leaf(RayState& st, const DevScene& sc, int 
         st.node = sc.nodes[0]; // restart from the root
         st.t_near = st.t_far;
         st.t_far = st.t_scene_far;
-        if (DBG) dbg->cnt[7]++;
+        if (DBG) {
+            dbg->cnt[7]++;
+            st.nidx = 0;
+        }
     } else {
         st.node = make_uint2(stk[64 * st.top], stk[64 * (depth + st.top)]);
+        if (DBG) st.nidx = stk[64 * (3 * depth + st.top)];
         const float tf = __uint_as_float(stk[64 * (2 * depth + st.top)]);
         if (st.top == 0) st.top = (uint32_t)depth;
         --st.top;
@@ -455,6 +480,8 @@ __device__ __forceinline__ void leaf(RayState& st, const DevScene& sc, RayDbg* d
     if (DBG) {
         dbg->cnt[1]++;
         dbg->cnt[2] += (int32_t)count;
+        if (dbg->t_pre)
+            for (uint32_t j = 0; j < count; ++j) dbg->t_pre[base + j] = 1;
     }
     const float o[3] = {st.ox, st.oy, st.oz}, d[3] = {st.dx, st.dy, st.dz};
     bool have_lb = false, done = false;
@@ -540,6 +567,10 @@ __device__ __forceinline__ void leaf(RayState& st, const DevScene& sc, RayDbg* d
                 const uint32_t j2 = two ? (uint32_t)__builtin_ctz(mask) : j1;
                 if (two) mask &= mask - 1;
                 const uint32_t s1 = w0 + j1, s2 = w0 + j2;
+                if (DBG && dbg->t_exact) {
+                    dbg->t_exact[base + s1] = 1;
+                    dbg->t_exact[base + s2] = 1;
+                }
                 const float4* r1p = sc.sgeom + 5 * (size_t)(base + s1);
                 const float4* r2p = sc.sgeom + 5 * (size_t)(base + s2);
                 const float4 a1 = r1p[0], b1 = r1p[1], c1 = r1p[2], e1 = r1p[3], f1 = r1p[4];
@@ -580,7 +611,10 @@ __device__ __forceinline__ void leaf(RayState& st, const DevScene& sc, RayDbg* d
                 done = true;
                 break;
             }
-            if (DBG) dbg->cnt[6]++;
+            if (DBG) {
+                dbg->cnt[6]++;
+                if (dbg->t_eval) dbg->t_eval[kid[i]] = 1;
+            }
             // particleResponse<4> and the skip rule (cudaRenderer.cu:1655-1671)
             const float gd = kg[i];
             const float gres = kd_expf(-0.0555555555556f * gd * gd);
@@ -604,6 +638,7 @@ __device__ __forceinline__ void leaf(RayState& st, const DevScene& sc, RayDbg* d
             st.A = st.A + alpha * (1.0f - st.A);
             if (DBG) {
                 dbg->cnt[4]++;
+                if (dbg->t_blend) dbg->t_blend[gid] = 1;
                 dbg->hash = (dbg->hash ^ (uint64_t)gid) * 0x100000001b3ull;
                 if (dbg->seq && dbg->nseq < dbg->seq_cap) dbg->seq[dbg->nseq] = gid;
                 dbg->nseq++;
@@ -623,8 +658,11 @@ __device__ __forceinline__ void leaf(RayState& st, const DevScene& sc, RayDbg* d
 
 // Per-wavefront LDS of the trace: the lanes' short-stack rings, depth
 // entries of 12 B (node words x, y and t_far, each entry-major over the 64
-// lanes): 768 * depth bytes per wavefront.
-__host__ __device__ constexpr size_t wave_lds_bytes(int depth) { return (size_t)depth * 768u; }
+// lanes): 768 * depth bytes per wavefront; the DBG variant adds a fourth
+// plane (the pushed nodes' indices), 1024 * depth bytes.
+__host__ __device__ constexpr size_t wave_lds_bytes(int depth, bool dbg = false) {
+    return (size_t)depth * (dbg ? 1024u : 768u);
+}
 
 // Whole ray: outer loop of cudaRenderer.cu:1552-1709.  CONE: the ray starts
 // at the view origin that sc.vcone was prepared for.  wave_lds: this wave's

@@ -230,6 +230,39 @@ KDGRT_API kdgrt_status kdgrt_render_debug(kdgrt_context* ctx, const kdgrt_scene*
                                           const kdgrt_camera_info* cam, int width, int height, int stack_depth,
                                           float* out_rgbt, int32_t* counters, uint64_t* id_hash,
                                           uint32_t* id_seq, int seq_cap, void* stream);
+/* Measurement helper, no reference counterpart: the VALU-issue ceiling.
+ * Launches k_valu_probe (eight independent fused multiply-add chains per
+ * lane, `iters` iterations, eight waves per SIMD) on `stream`; the caller
+ * times it.  *waves and *fma_per_wave give the instruction count: the
+ * measured rate waves * fma_per_wave / time is the peak VALU instruction
+ * rate that a trace kernel's SQ_INSTS_VALU count is compared with. */
+KDGRT_API kdgrt_status kdgrt_valu_probe(kdgrt_context* ctx, int iters, void* stream, uint64_t* waves,
+                                        uint64_t* fma_per_wave);
+/* Measurement helper, no reference counterpart (the compulsory bytes of a
+ * launch): the debug kernel marks, with a byte set to 1, every record a ray
+ * reads.  Arrays are caller-owned device memory that the caller zeroes; any
+ * may be NULL.  nodes / quads: n_nodes bytes (node records visited -- the
+ * reference's 8-B node fetches -- and child-pair quad records loaded, by
+ * first-child index); slots_pretested / slots_tested: n_offsets bytes (leaf
+ * slots through the pre-test and through the exact test); gaussians_evaluated
+ * / gaussians_blended: n_gaussians bytes (IDs entering the composite loop --
+ * the reference's 256-B fetch_gaussian -- and IDs blended, whose SH record
+ * is read). */
+typedef struct kdgrt_touch_maps {
+    uint8_t* nodes;
+    uint8_t* quads;
+    uint8_t* slots_pretested;
+    uint8_t* slots_tested;
+    uint8_t* gaussians_evaluated;
+    uint8_t* gaussians_blended;
+} kdgrt_touch_maps;
+KDGRT_API kdgrt_status kdgrt_render_touch(kdgrt_context* ctx, const kdgrt_scene* scene,
+                                          const kdgrt_camera_info* cam, int width, int height, int stack_depth,
+                                          float* out_rgbt, const kdgrt_touch_maps* maps, void* stream);
+KDGRT_API kdgrt_status kdgrt_render_rays_touch(kdgrt_context* ctx, const kdgrt_scene* scene,
+                                               const float* origins, const float* dirs, uint64_t n_rays,
+                                               int stack_depth, float* out_rgbt, const kdgrt_touch_maps* maps,
+                                               void* stream);
 /* Measurement helper, no reference counterpart: the plain kernel with
  * shader-clock marks, writing per ray (y*width+x) eight uint64: cycle sums
  * for descend, pre-test windows, exact tests + hit buffer, composite, stack

@@ -243,9 +243,20 @@ inline void ray_prim(const V3<T>& o, const V3<T>& d, uint32_t id, const Gauss256
 // 1538-1541, 1612-1616): color and A are the running values passed in by
 // reference and not reset, t_scene_far is clamped to max_t, and the composite
 // loop stops at a hit beyond max_t.
+// Records a ray reads, as byte maps (any may be null): node records visited
+// (by index), leaf slots tested, leaf slots whose test accepted, Gaussians
+// entering the composite loop, Gaussians blended.
+struct Touch {
+    uint8_t* node;
+    uint8_t* slot;
+    uint8_t* hit_slot;
+    uint8_t* eval;
+    uint8_t* blend;
+};
+
 template <class T>
 void trace(const Scene& sc, V3<T> o, V3<T> d, int D, V3<T>& color, T& A, RayOut& ro, uint32_t* seq, int seq_cap,
-           bool hybrid = false, T max_t = T(0)) {
+           bool hybrid = false, T max_t = T(0), const Touch* tc = nullptr) {
     std::memset(ro.cnt, 0, sizeof(ro.cnt));
     ro.hash = kFnvBasis;
     int nseq = 0;
@@ -276,6 +287,7 @@ void trace(const Scene& sc, V3<T> o, V3<T> d, int D, V3<T>& color, T& A, RayOut&
     unsigned top = (unsigned)D - 1, quant = 0;
 
     Node node = sc.nodes[0];
+    uint32_t cur = 0; // index of node (touch maps)
     T t_near = t_scene_near, t_far = t_scene_far;
     std::vector<HitRec<T>> hits;
     const T po[3] = {o.x, o.y, o.z}, pd[3] = {d.x, d.y, d.z};
@@ -298,14 +310,20 @@ void trace(const Scene& sc, V3<T> o, V3<T> d, int D, V3<T>& color, T& A, RayOut&
                 t_far = t_split;
             }
             ro.cnt[0]++; // node_visits
+            if (tc && tc->node) tc->node[cur] = 1;
             node = sc.nodes[idx];
+            cur = idx;
         }
+        if (tc && tc->node) tc->node[cur] = 1;
         const uint32_t base = node.y, count = node.x >> 3;
         hits.clear();
         ro.cnt[1]++; // leaf_visits
         for (uint32_t k = 0; k < count; ++k) {
             const uint32_t id = sc.offsets[base + k];
+            const size_t before = hits.size();
             ray_prim<T>(o, d, id, sc.g[id], t_near, t_far, hits);
+            if (tc && tc->slot) tc->slot[base + k] = 1;
+            if (tc && tc->hit_slot && hits.size() > before) tc->hit_slot[base + k] = 1;
         }
         ro.cnt[2] += (int32_t)count;       // intersection_tests
         ro.cnt[3] += (int32_t)hits.size(); // hits_found
@@ -325,6 +343,7 @@ void trace(const Scene& sc, V3<T> o, V3<T> d, int D, V3<T>& color, T& A, RayOut&
             if (hybrid && hits[i].t > max_t) break;
             ro.cnt[6]++; // N_eval
             const uint32_t gid = hits[i].id;
+            if (tc && tc->eval) tc->eval[gid] = 1;
             const Gauss256& g = sc.g[gid];
             const V3<T> ppos = {T(g.pos[0]), T(g.pos[1]), T(g.pos[2])};
             const V3<T> giscl = {T(g.scale[0]), T(g.scale[1]), T(g.scale[2])};
@@ -346,6 +365,7 @@ void trace(const Scene& sc, V3<T> o, V3<T> d, int D, V3<T>& color, T& A, RayOut&
             color = add(color, muls(muls(c, sample_opacity), T(1) - A));
             A = A + sample_opacity * (T(1) - A);
             ro.cnt[4]++; // blend_ops
+            if (tc && tc->blend) tc->blend[gid] = 1;
             ro.hash = (ro.hash ^ (uint64_t)gid) * kFnvPrime;
             if (seq && nseq < seq_cap) seq[nseq] = gid;
             nseq++;
@@ -354,11 +374,13 @@ void trace(const Scene& sc, V3<T> o, V3<T> d, int D, V3<T>& color, T& A, RayOut&
         if ((A > T(0.99f)) | (t_far >= t_scene_far)) break;
         if (quant == 0) {
             node = sc.nodes[0]; // restart from the root
+            cur = 0;
             t_near = t_far;
             t_far = t_scene_far;
             ro.cnt[7]++;
         } else {
             const uint32_t nid = st_node[top];
+            cur = nid;
             const T tm = st_t[top];
             if (top == 0) top = (unsigned)D;
             --top;
@@ -383,7 +405,7 @@ template <class T> inline void make_ray(const CamInfo& c, int x, int y, V3<T>& o
 
 template <class T>
 void render_view(const Scene& sc, const CamInfo& cam, int W, int H, int D, T* out_rgbt, int32_t* counters,
-                 uint64_t* hashes, uint32_t* seq, int seq_cap, int threads) {
+                 uint64_t* hashes, uint32_t* seq, int seq_cap, int threads, const Touch* tc = nullptr) {
     if (threads > 0) omp_set_num_threads(threads);
     const int tiles_x = (W + 15) / 16, tiles_y = (H + 15) / 16;
 #pragma omp parallel for schedule(dynamic)
@@ -396,7 +418,7 @@ void render_view(const Scene& sc, const CamInfo& cam, int W, int H, int D, T* ou
                 RayOut ro;
                 const size_t rid = (size_t)y * W + x;
                 make_ray<T>(cam, x, y, o, d);
-                trace<T>(sc, o, d, D, c, A, ro, seq ? seq + rid * seq_cap : nullptr, seq_cap);
+                trace<T>(sc, o, d, D, c, A, ro, seq ? seq + rid * seq_cap : nullptr, seq_cap, false, T(0), tc);
                 const size_t pid = (size_t)(H - y - 1) * W + x; // framebuffer row flip (cudaRenderer.cu:2367)
                 if (out_rgbt) {
                     out_rgbt[4 * pid + 0] = c.x;
@@ -412,7 +434,7 @@ void render_view(const Scene& sc, const CamInfo& cam, int W, int H, int D, T* ou
 
 template <class T>
 void render_rays(const Scene& sc, const float* origins, const float* dirs, uint64_t n, int D, T* out_rgbt,
-                 int32_t* counters, uint64_t* hashes, int threads) {
+                 int32_t* counters, uint64_t* hashes, int threads, const Touch* tc = nullptr) {
     if (threads > 0) omp_set_num_threads(threads);
 #pragma omp parallel for schedule(dynamic, 256)
     for (long long i = 0; i < (long long)n; ++i) {
@@ -421,7 +443,7 @@ void render_rays(const Scene& sc, const float* origins, const float* dirs, uint6
         V3<T> c;
         T A;
         RayOut ro;
-        trace<T>(sc, o, d, D, c, A, ro, nullptr, 0);
+        trace<T>(sc, o, d, D, c, A, ro, nullptr, 0, false, T(0), tc);
         if (out_rgbt) {
             out_rgbt[4 * i + 0] = c.x;
             out_rgbt[4 * i + 1] = c.y;
@@ -786,6 +808,25 @@ void oracle_ray_dirs(const void* cam, int W, int H, float* dirs) {
             p[2] = d.z;
         }
 }
+// Touch maps of a view or a ray list (FP32): node records visited, leaf
+// slots tested, leaf slots accepted, Gaussians evaluated, Gaussians blended;
+// byte maps the caller zeroes.  Several rays store the same value 1 into a
+// byte, so the concurrent stores agree.
+void oracle_touch_view_f32(const void* nodes, const uint32_t* offsets, const void* g, const float* aabb,
+                           const void* cam, int W, int H, int D, uint8_t* node, uint8_t* slot, uint8_t* hit_slot,
+                           uint8_t* eval, uint8_t* blend, int threads) {
+    const Touch tc{node, slot, hit_slot, eval, blend};
+    render_view<float>(make_scene(nodes, offsets, g, aabb), *(const CamInfo*)cam, W, H, D, nullptr, nullptr, nullptr,
+                       nullptr, 0, threads, &tc);
+}
+void oracle_touch_rays_f32(const void* nodes, const uint32_t* offsets, const void* g, const float* aabb,
+                           const float* origins, const float* dirs, uint64_t n, int D, uint8_t* node, uint8_t* slot,
+                           uint8_t* hit_slot, uint8_t* eval, uint8_t* blend, int threads) {
+    const Touch tc{node, slot, hit_slot, eval, blend};
+    render_rays<float>(make_scene(nodes, offsets, g, aabb), origins, dirs, n, D, nullptr, nullptr, nullptr, threads,
+                       &tc);
+}
+
 // Unit entry points of the intersection and SH restatements, for the
 // closed-form pins.  g: one 256-B record with inverted scales.
 int oracle_ray_gaussian_f32(const float* o, const float* d, const void* g, float t_near, float t_far, float* out) {

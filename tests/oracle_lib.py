@@ -207,3 +207,26 @@ def eval_sh(d, g_rec, f64=False):
         out = np.zeros(3, np.float32)
         lib().oracle_eval_sh_f32(_p(np.asarray(d, np.float32)), _p(g), _p(out))
     return out
+
+
+def touch(nodes, offsets, g_inv, aabb, n_gaussians, ci=None, w=0, h=0, origins=None, dirs=None, depth=12):
+    """Touch maps of a view (ci, w, h) or a ray list (origins, dirs): bool
+    arrays node (n_nodes), slot / hit_slot (n_offsets), eval / blend
+    (n_gaussians)."""
+    nodes = np.ascontiguousarray(nodes)
+    offsets = np.ascontiguousarray(offsets, np.uint32)
+    aabb = np.ascontiguousarray(aabb, np.float32)
+    m = {"node": np.zeros(len(nodes), np.uint8), "slot": np.zeros(max(1, len(offsets)), np.uint8),
+         "hit_slot": np.zeros(max(1, len(offsets)), np.uint8), "eval": np.zeros(n_gaussians, np.uint8),
+         "blend": np.zeros(n_gaussians, np.uint8)}
+    maps = [_p(m[k]) for k in ("node", "slot", "hit_slot", "eval", "blend")]
+    if ci is not None:
+        cam = np.frombuffer(bytes(ci), dtype=np.uint8).copy()
+        lib().oracle_touch_view_f32(_p(nodes), _p(offsets), _p(g_inv), _p(aabb), _p(cam), C.c_int(w), C.c_int(h),
+                                    C.c_int(depth), *maps, C.c_int(0))
+    else:
+        origins = np.ascontiguousarray(origins, np.float32)
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        lib().oracle_touch_rays_f32(_p(nodes), _p(offsets), _p(g_inv), _p(aabb), _p(origins), _p(dirs),
+                                    C.c_uint64(len(origins)), C.c_int(depth), *maps, C.c_int(0))
+    return {k: v[:len(offsets)].astype(bool) if k in ("slot", "hit_slot") else v.astype(bool) for k, v in m.items()}
