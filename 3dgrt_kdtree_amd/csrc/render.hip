@@ -811,6 +811,18 @@ struct kdgrt_batch {
     hipGraphExec_t graph_exec = nullptr;
     void* graph_stream = nullptr;
     std::vector<void*> d_perms; // ray-key order of the long ray sets
+    // the ray-key sort of each long ray set, rerun (eagerly, before the
+    // persistent launch) by every kdgrt_batch_run: workspace allocated here
+    struct SortJob {
+        DevScene sc;
+        const float* ro;
+        const float* rd;
+        uint32_t n;
+        uint32_t* perm;
+        void* ws;
+        size_t temp;
+    };
+    std::vector<SortJob> sorts;
 };
 
 using kdgrt::set_error;
@@ -1750,18 +1762,24 @@ static void ray_key_box(const DevScene& sc, float3& lo, float3& inv) {
 
 static uint32_t ray_key_grid(uint32_t n) { return (uint32_t)std::min<uint64_t>(((uint64_t)n + 255) / 256, 8192); }
 
-// Ray-key order of a ray list into perm (n entries) on stream s.  The
-// scratch (keys, indices, sort temporaries) comes from the stream-ordered
-// allocator and is released on s after the sort.
-static kdgrt_status ray_order(const DevScene& sc, const float* ro, const float* rd, uint32_t n, uint32_t* perm,
-                              hipStream_t s) {
-    size_t temp = 0;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                               (const uint32_t*)nullptr, (uint32_t*)nullptr, n, 0, kRayKeyBits, s));
+// Workspace of the ray-key sort of n rays: keys in and out and indices in
+// (256-B aligned arrays of n words each), then the radix sort's temporaries
+// (*temp bytes).
+static kdgrt_status ray_order_ws_bytes(uint32_t n, size_t* temp, size_t* total) {
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, *temp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                               (const uint32_t*)nullptr, (uint32_t*)nullptr, n, 0, kRayKeyBits,
+                                               (hipStream_t)nullptr));
     const size_t arr = ((size_t)n * 4 + 255) & ~(size_t)255;
-    void* block = nullptr;
-    HIP_TRY(hipMallocAsync(&block, 3 * arr + temp, s));
-    char* base = (char*)block;
+    *total = 3 * arr + *temp;
+    return KDGRT_OK;
+}
+
+// Ray-key order of a ray list into perm (n entries) on stream s, in a
+// workspace of ray_order_ws_bytes(n).
+static kdgrt_status ray_order_ws(const DevScene& sc, const float* ro, const float* rd, uint32_t n, uint32_t* perm,
+                                 void* ws, size_t temp, hipStream_t s) {
+    const size_t arr = ((size_t)n * 4 + 255) & ~(size_t)255;
+    char* base = (char*)ws;
     uint32_t* k_in = (uint32_t*)base;
     uint32_t* k_out = (uint32_t*)(base + arr);
     uint32_t* v_in = (uint32_t*)(base + 2 * arr);
@@ -1770,10 +1788,22 @@ static kdgrt_status ray_order(const DevScene& sc, const float* ro, const float* 
     ray_key_box(sc, lo, inv);
     hipLaunchKernelGGL(k_ray_keys, dim3(ray_key_grid(n)), dim3(256), 0, s, ro, rd, n, lo, inv, k_in, v_in);
     HIP_TRY(hipGetLastError());
-    const hipError_t e = hipcub::DeviceRadixSort::SortPairs(tmp, temp, k_in, k_out, v_in, perm, n, 0, kRayKeyBits, s);
-    HIP_TRY(hipFreeAsync(block, s));
-    HIP_TRY(e);
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, temp, k_in, k_out, v_in, perm, n, 0, kRayKeyBits, s));
     return KDGRT_OK;
+}
+
+// Ray-key order with a workspace from the stream-ordered allocator,
+// released on s after the sort.
+static kdgrt_status ray_order(const DevScene& sc, const float* ro, const float* rd, uint32_t n, uint32_t* perm,
+                              hipStream_t s) {
+    size_t temp = 0, total = 0;
+    kdgrt_status st = ray_order_ws_bytes(n, &temp, &total);
+    if (st != KDGRT_OK) return st;
+    void* block = nullptr;
+    HIP_TRY(hipMallocAsync(&block, total, s));
+    st = ray_order_ws(sc, ro, rd, n, perm, block, temp, s);
+    HIP_TRY(hipFreeAsync(block, s));
+    return st;
 }
 
 static kdgrt_status launch_rays(kdgrt_context* ctx, const kdgrt_scene* sc, const float* ro, const float* rd,
@@ -1917,6 +1947,12 @@ kdgrt_status kdgrt_batch_run(kdgrt_batch* b, void* stream, int use_graph) {
     if (!b) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "batch_run: null");
     hipStream_t s = (hipStream_t)stream;
     HIP_TRY(hipSetDevice(b->ctx->device));
+    // ray-key order of the long ray sets, on the run's stream ahead of the
+    // launch (outside any captured graph: the sort's library calls stay out)
+    for (const auto& j : b->sorts) {
+        const kdgrt_status st = ray_order_ws(j.sc, j.ro, j.rd, j.n, j.perm, j.ws, j.temp, s);
+        if (st != KDGRT_OK) return st;
+    }
     if (!use_graph) return kdgrt::batch_launch(b, s);
     if (b->graph_exec && b->graph_stream != stream) {
         hipGraphExecDestroy(b->graph_exec);
@@ -1963,6 +1999,7 @@ kdgrt_status kdgrt_batch_destroy(kdgrt_batch* b) {
     hipFree(b->d_items);
     hipFree(b->d_counters);
     for (void* p : b->d_perms) hipFree(p);
+    for (auto& j : b->sorts) hipFree(j.ws);
     delete b;
     return KDGRT_OK;
 }
@@ -2046,20 +2083,23 @@ kdgrt_status batch_build(kdgrt_context* ctx, const kdgrt_view* views, int n_view
     b->n_rays = rays;
     hipError_t e = hipSuccess;
     {
-        // long ray sets are traced in ray-key order, fixed here (the order
-        // changes no result; it stays valid whatever the rays hold later)
+        // long ray sets are traced in ray-key order; every run sorts them
+        // again (the order changes no result), into buffers allocated here
         for (WorkItem& w : items) {
             if (w.W != 0 || w.n_rays < kRaySortMin || w.n_rays > 0x7FFFFFFFull) continue;
             void* perm = nullptr;
             if ((e = hipMalloc(&perm, 4 * w.n_rays)) != hipSuccess) break;
             b->d_perms.push_back(perm);
-            if (ray_order(w.sc, w.ro, w.rd, (uint32_t)w.n_rays, (uint32_t*)perm, nullptr) != KDGRT_OK) {
+            kdgrt_batch::SortJob j{w.sc, w.ro, w.rd, (uint32_t)w.n_rays, (uint32_t*)perm, nullptr, 0};
+            size_t total = 0;
+            if (ray_order_ws_bytes(j.n, &j.temp, &total) != KDGRT_OK) {
                 e = hipErrorUnknown;
                 break;
             }
+            if ((e = hipMalloc(&j.ws, total)) != hipSuccess) break;
+            b->sorts.push_back(j);
             w.perm = (const uint32_t*)perm;
         }
-        if (e == hipSuccess) e = hipDeviceSynchronize();
     }
     if (e == hipSuccess) e = hipMalloc(&b->d_items, items.size() * sizeof(WorkItem));
     if (e == hipSuccess) e = hipMalloc(&b->d_counters, 64 * sizeof(uint32_t));
@@ -2069,6 +2109,7 @@ kdgrt_status batch_build(kdgrt_context* ctx, const kdgrt_view* views, int n_view
         hipFree(b->d_items);
         hipFree(b->d_counters);
         for (void* p : b->d_perms) hipFree(p);
+        for (auto& j : b->sorts) hipFree(j.ws);
         delete b;
         return set_error(KDGRT_ERR_DEVICE, std::string("batch_create: ") + hipGetErrorString(e));
     }

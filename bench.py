@@ -3,9 +3,15 @@
 Headline line (one JSON object on stdout): the plain one-view launch over the
 1M-Gaussian clustered scene at 1920x1080 (1 ray/pixel), Mrays/s with the
 scene resident in HBM, plus
-  roofline      algorithmic bytes per launch / mean kernel time vs 8 TB/s,
+  roofline      compulsory bytes of a render call (every record its rays read,
+                counted once, from the device touch maps) / mean call time vs
+                8 TB/s; the per-ray logical bytes of the reference layout and
+                the measured HBM traffic beside it,
+  valu          the VALU-issue rate of k_view against the measured ceiling of
+                a pure fused-multiply-add kernel,
   cpu_baseline  the FP32 oracle (CPU restatement of the reference
-                traversal+compositing, OpenMP) on this host's cores,
+                traversal+compositing, OpenMP) on this host's cores, and the
+                10k-Gaussian 256x256 case the reference runs on the CPU,
   batched       the 64 mixed-resolution views in one persistent launch
                 (hipGraph replay) against 64 plain launches of the same views,
   hbm_copy      a 1 GiB streaming copy (k_copy), the measured HBM ceiling and
@@ -83,16 +89,24 @@ def algorithmic_bytes(counters):
     return int(8 * (c[:, 0] + c[:, 1]).sum() + 52 * c[:, 2].sum() + 256 * c[:, 6].sum() + 16 * len(c))
 
 
-def load_traffic(workload_key):
-    """HBM bytes per launch of the dominant kernel for `workload_key`, from the
-    calibrated FETCH_SIZE / WRITE_SIZE counter passes summarised in
-    profiles/traffic.json (None when no summary exists)."""
-    p = os.path.join(ROOT, "profiles", "traffic.json")
+def load_profile(name, workload_key):
+    """One entry of a counter summary under profiles/ (None when absent)."""
+    p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
         return None
     with open(p) as f:
-        entry = json.load(f).get(workload_key)
-    return None if entry is None else int(entry["hbm_bytes_per_launch"])
+        return json.load(f).get(workload_key)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def copy_bandwidth(dev, stream, gib=1, reps=5):
@@ -225,7 +239,7 @@ def multi_scene_batch(dev, scenes, aabbs, counts, tuning, n_views=256, n_rays=4_
         rays_total += per
     stream = dev.stream()
 
-    def run_plain():
+    def run_plain(sync_each=True):
         for p in plain:
             if p[0] == "view":
                 _, sc, ci, w, h, b = p
@@ -234,13 +248,19 @@ def multi_scene_batch(dev, scenes, aabbs, counts, tuning, n_views=256, n_rays=4_
                 _, sc, ob, db, n, out = p
                 K.check(K.lib().kdgrt_render_rays(dev.h, sc.h, ob.ptr, db.ptr, K.C.c_uint64(n), 12, out.ptr,
                                                   stream))
-            dev.sync(stream)
+            if sync_each:
+                dev.sync(stream)
+        dev.sync(stream)
 
     run_plain()
     t0 = time.perf_counter()
     for _ in range(reps):
         run_plain()
     plain_s = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run_plain(sync_each=False)
+    plain_nosync_s = (time.perf_counter() - t0) / reps
     batch = K.Batch(dev, views, sets, tuning)
     batch.run(stream, use_graph=True)
     dev.sync(stream)
@@ -255,7 +275,10 @@ def multi_scene_batch(dev, scenes, aabbs, counts, tuning, n_views=256, n_rays=4_
     out = {"scenes": len(scenes), "gaussians": [int(c) for c in counts], "views": n_views,
            "incoherent_rays": per * len(scenes), "rays": rays_total,
            "plain_mrays_s": round(rays_total / plain_s / 1e6, 2),
+           "plain_nosync_mrays_s": round(rays_total / plain_nosync_s / 1e6, 2),
            "batched_graph_mrays_s": round(rays_total / batch_s / 1e6, 2), "speedup": round(plain_s / batch_s, 3),
+           "speedup_vs_nosync": round(plain_nosync_s / batch_s, 3),
+           "note": "every batched run sorts its ray sets into ray-key order before the launch (timed)",
            "tuning": tuning.as_dict()}
     log(f"multi-scene batch: {out}")
     return out
@@ -363,7 +386,11 @@ def tile_order_legs(dev, sc, ci, aabb, W, H, out, stream, steps):
 
 
 def cpu_baseline(tree, g_inv, ci, w, h, budget_s):
+    """The FP32 oracle with OpenMP on this host: the headline view (median of
+    up to five full renders within budget_s), and the 10k-Gaussian 256x256
+    case (uniform cube, eye (0,0,3)) that the reference runs on the CPU."""
     import oracle_lib as O
+    import scenes as S
     threads = O.lib().oracle_max_threads()
     nodes, offs, aabb = tree.nodes, tree.offsets, tree.aabb
     times = []
@@ -373,9 +400,25 @@ def cpu_baseline(tree, g_inv, ci, w, h, budget_s):
         O.render(nodes, offs, g_inv, aabb, ci, w, h, threads=threads)
         times.append(time.time() - t0)
     med = float(np.median(times))
+    g0 = K.generate_gaussians(0, 10000, 1000)
+    t0 = time.time()
+    tree0 = K.Tree.build(g0)
+    build0 = time.time() - t0
+    gi0, ci0 = K.invert_scales(g0), S.cam_c0(256, 256)
+    t0s = []
+    for _ in range(5):
+        t0 = time.time()
+        O.render(tree0.nodes, tree0.offsets, gi0, tree0.aabb, ci0, 256, 256, threads=threads)
+        t0s.append(time.time() - t0)
+    med0 = float(np.median(t0s))
+    tree0.free()
     return {"value": round(w * h / med / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": f"full {w}x{h} view of the same 1M scene, FP32 oracle with OpenMP dynamic 16x16 tiles, "
-                      f"median of {len(times)} run(s)"}
+                      f"median of {len(times)} run(s)",
+            "config0": {"workload": "10k Gaussians uniform in [-1,1]^3, 256x256 view from (0,0,3), fovy 60",
+                        "mrays_s": round(256 * 256 / med0 / 1e6, 4), "ms": round(med0 * 1e3, 3),
+                        "host_sah_build_s": round(build0, 3), "threads": threads}}
 
 
 def main():
@@ -434,13 +477,29 @@ def main():
     dbg = dev.render_debug(sc, ci, W, H)
     cnt = dbg["counters"]
     B = algorithmic_bytes(cnt)
-    achieved = B / (ms_launch * 1e-3) / 1e9
     mean_c = cnt.reshape(-1, K.N_COUNTERS).mean(0)
     log("mean counters/ray: " + ", ".join(f"{n}={v:.2f}" for n, v in zip(K.COUNTER_NAMES, mean_c)))
+    # Compulsory bytes of one render call: k_view's records read once each
+    # (touch maps of the debug kernel, same rays and tree) plus k_cone_prep's
+    # pass over the leaf slots (16-B bound sphere read, 16-B cone record written).
+    touch = dev.render_touch(sc, ci, W, H)
+    comp = K.compulsory_bytes(touch, tree.offsets, rays)
+    cone_prep = 2 * 16 * (int(st["n_offsets"]) + 4)
+    comp_launch = comp["this_layout"] + cone_prep
+    achieved = comp_launch / (ms_launch * 1e-3) / 1e9
+    traffic = load_profile("traffic.json", "plain_1080p_1M")
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": load_traffic("plain_1080p_1M"),
-            "algorithmic_bytes_per_launch": B, "bytes_per_ray": round(B / rays, 1),
-            "kernel": "k_view<8, false, 0, false, true> (timed with its k_cone_prep, which also sorts the tile order)"}
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": None if traffic is None else int(traffic["hbm_bytes_per_launch"]),
+            "traffic_source": None if traffic is None else traffic.get("source"),
+            "compulsory_bytes_per_launch": comp_launch,
+            "compulsory_parts": {**comp["this_layout_parts"], "cone_prep": cone_prep},
+            "compulsory_reference_layout_bytes": comp["reference_layout"],
+            "logical_bytes_per_launch": B, "logical_bytes_per_ray": round(B / rays, 1),
+            "logical_GB_s": round(B / (ms_launch * 1e-3) / 1e9, 1),
+            "kernel": "k_cone_prep + k_view<8, false, 0, false, true> (one render call; k_cone_prep also sorts the "
+                      "tile order)"}
+    log(f"roofline: {roof}")
 
     phases = None
     if args.phases:
@@ -462,6 +521,17 @@ def main():
 
     copy = copy_bandwidth(dev, stream)
     log(f"hbm copy: {copy}")
+    roof["frac_of_copy_ceiling"] = round(achieved / copy["GB_s"], 5)
+    probe = dev.valu_probe(stream=stream)
+    vprof = load_profile("valu.json", "plain_1080p_1M")
+    valu = {"probe": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in probe.items()},
+            "kernel_valu_instr_per_launch": None, "frac": None}
+    if vprof is not None:
+        kv = float(vprof["sq_insts_valu_per_dispatch"])
+        valu.update({"kernel_valu_instr_per_launch": kv, "source": vprof.get("source"),
+                     "kernel_valu_instr_per_s": round(kv / (ms_launch * 1e-3), 1),
+                     "frac": round(kv / (ms_launch * 1e-3) / probe["valu_instr_per_s"], 4)})
+    log(f"valu: {valu}")
     result = {"metric": METRIC, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
               "warmup": args.warmup, "ms_per_step": round(ms_launch, 4), "higher_is_better": True, "scaling": "weak",
               "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -471,7 +541,7 @@ def main():
                                      f"previous launch's tile durations",
                          "gaussians": len(g), "width": W, "height": H, "stack_depth": 12,
                          "tile_order": "frame-to-frame schedule (raster-order rate under tile_order)"},
-              "device": info["name"], "roofline": roof, "hbm_copy": copy,
+              "device": info["name"], "roofline": roof, "valu": valu, "hbm_copy": copy,
               **({"phases": phases} if phases else {}),
               "build": {"host_sah_build_s": round(build_s, 2), "threads": os.cpu_count(), "nodes": st["n_nodes"],
                         "leaf_refs": st["n_offsets"], "proxy_triangles": st["n_triangles"]}}
@@ -502,6 +572,13 @@ def main():
                 dev.render(sc, vci, w, h, b, stream=stream)
                 dev.sync(stream)
         plain_s = (time.perf_counter() - t0) / reps
+        # the same launches queued back to back, one wait at the end
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for (vci, w, h), b in zip(views, bufs):
+                dev.render(sc, vci, w, h, b, stream=stream)
+            dev.sync(stream)
+        plain_nosync_s = (time.perf_counter() - t0) / reps
         # batched: autotune, then hipGraph replays of one persistent launch
         tun, tms = K.autotune(dev, vlist, max_trials=20, reps=2)
         batch = K.Batch(dev, vlist, (), tun)
@@ -515,8 +592,10 @@ def main():
         result["batched"] = {
             "views": 64, "rays": total_rays,
             "plain_per_view_mrays_s": round(total_rays / plain_s / 1e6, 2),
+            "plain_per_view_nosync_mrays_s": round(total_rays / plain_nosync_s / 1e6, 2),
             "batched_graph_mrays_s": round(total_rays / batch_s / 1e6, 2),
-            "speedup": round(plain_s / batch_s, 3), "tuning": tun.as_dict(), "autotune_best_ms": round(tms, 3)}
+            "speedup": round(plain_s / batch_s, 3), "speedup_vs_nosync": round(plain_nosync_s / batch_s, 3),
+            "tuning": tun.as_dict(), "autotune_best_ms": round(tms, 3)}
         log(f"64 views ({total_rays} rays): plain {plain_s * 1e3:.2f} ms, batched {batch_s * 1e3:.2f} ms")
         batch.free()
         for b in bufs:
