@@ -385,6 +385,33 @@ def tile_order_legs(dev, sc, ci, aabb, W, H, out, stream, steps):
     return r
 
 
+def rays_leg(dev, sc, aabb, n, stream, steps, warmup):
+    """kdgrt_render_rays over n rays with origins uniform in the scene box and
+    directions uniform on the sphere (the ray-key sort included)."""
+    rng = np.random.default_rng(1005)
+    lo, hi = aabb[:3].astype(np.float64), aabb[3:].astype(np.float64)
+    o = (lo + (hi - lo) * rng.random((n, 3))).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    ob, db, out = dev.upload(o), dev.upload(d), dev.alloc(n * 16)
+
+    def launch():
+        K.check(K.lib().kdgrt_render_rays(dev.h, sc.h, ob.ptr, db.ptr, K.C.c_uint64(n), 12, out.ptr, stream))
+    for _ in range(warmup):
+        launch()
+    dev.sync()
+    dev.timer_begin(stream)
+    for _ in range(steps):
+        launch()
+    ms = dev.timer_end(stream) / steps
+    dev.sync()
+    for b in (ob, db, out):
+        b.free()
+    r = {"rays": n, "ms_per_call": round(ms, 4), "mrays_s": round(n / (ms * 1e-3) / 1e6, 2)}
+    log(f"incoherent rays: {r}")
+    return r
+
+
 def cpu_baseline(tree, g_inv, ci, w, h, budget_s):
     """The FP32 oracle with OpenMP on this host: the headline view (median of
     up to five full renders within budget_s), and the 10k-Gaussian 256x256
@@ -439,6 +466,9 @@ def main():
                     help="how long a busy device waits to fill a batch (0: launch as soon as a slot is free)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--phases", action="store_true", help="add the per-phase cycle profile of the plain kernel")
+    ap.add_argument("--rays", type=int, default=0,
+                    help="also time kdgrt_render_rays over this many incoherent rays on the 1M scene (0: skip)")
+    ap.add_argument("--no-autotune", action="store_true", help="batched leg with the default tuning")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); this build has no multi-GPU path.",
@@ -522,15 +552,20 @@ def main():
     copy = copy_bandwidth(dev, stream)
     log(f"hbm copy: {copy}")
     roof["frac_of_copy_ceiling"] = round(achieved / copy["GB_s"], 5)
+    # VALU roofline: lane operations per SIMD-cycle of k_view against those of
+    # k_valu_probe (every lane active, issue-limited), both from the counter
+    # runs summarised in profiles/valu.json; the live probe rate is beside it.
     probe = dev.valu_probe(stream=stream)
     vprof = load_profile("valu.json", "plain_1080p_1M")
-    valu = {"probe": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in probe.items()},
-            "kernel_valu_instr_per_launch": None, "frac": None}
+    valu = {"bound": "valu", "probe_live": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in probe.items()},
+            "frac": None}
     if vprof is not None:
-        kv = float(vprof["sq_insts_valu_per_dispatch"])
-        valu.update({"kernel_valu_instr_per_launch": kv, "source": vprof.get("source"),
-                     "kernel_valu_instr_per_s": round(kv / (ms_launch * 1e-3), 1),
-                     "frac": round(kv / (ms_launch * 1e-3) / probe["valu_instr_per_s"], 4)})
+        valu.update({"lane_op_rate": round(vprof["lane_op_rate"], 5),
+                     "probe_lane_op_rate": round(vprof["probe_lane_op_rate"], 5),
+                     "frac": round(vprof["lane_op_rate"] / vprof["probe_lane_op_rate"], 4),
+                     "issue_rate": round(vprof["issue_rate"], 4), "probe_issue_rate": round(vprof["probe_issue_rate"], 4),
+                     "kernel_valu_instr_per_launch": vprof["sq_insts_valu_per_dispatch"],
+                     "source": vprof.get("source")})
     log(f"valu: {valu}")
     result = {"metric": METRIC, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
               "warmup": args.warmup, "ms_per_step": round(ms_launch, 4), "higher_is_better": True, "scaling": "weak",
@@ -580,7 +615,10 @@ def main():
             dev.sync(stream)
         plain_nosync_s = (time.perf_counter() - t0) / reps
         # batched: autotune, then hipGraph replays of one persistent launch
-        tun, tms = K.autotune(dev, vlist, max_trials=20, reps=2)
+        if args.no_autotune:
+            tun, tms = K.default_tuning(dev), float("nan")
+        else:
+            tun, tms = K.autotune(dev, vlist, max_trials=20, reps=2)
         batch = K.Batch(dev, vlist, (), tun)
         batch.run(stream, use_graph=True)
         dev.sync(stream)
@@ -600,6 +638,9 @@ def main():
         batch.free()
         for b in bufs:
             b.free()
+
+    if args.rays:
+        result["incoherent_rays"] = rays_leg(dev, sc, aabb, args.rays, stream, args.steps, args.warmup)
 
     if not (args.no_serve and args.no_multi):
         tun = K.default_tuning(dev)
