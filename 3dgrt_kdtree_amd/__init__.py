@@ -109,7 +109,9 @@ class RaySet(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch_views", C.c_int32),
                 ("max_inflight_batches", C.c_int32), ("plain_launches", C.c_int32), ("tuning", Tuning),
-                ("batch_window_ms", C.c_double), ("simulate_mrays_per_s", C.c_double)]
+                ("batch_window_ms", C.c_double), ("simulate_mrays_per_s", C.c_double),
+                ("persistent_kernel", C.c_int32), ("persistent_max_requests", C.c_int32),
+                ("epoch_idle_ms", C.c_double), ("persistent_spin_limit", C.c_uint32)]
 
 
 class Request(C.Structure):
@@ -748,7 +750,8 @@ class Server:
     scheduler against a modelled device."""
 
     def __init__(self, dev, budget_bytes, max_batch_views=16, max_inflight=2, plain=False, tuning=None,
-                 window_ms=1.0, simulate_mrays_per_s=0.0):
+                 window_ms=1.0, simulate_mrays_per_s=0.0, persistent=False, epoch_max_requests=0, epoch_idle_ms=5.0,
+                 spin_limit=0):
         cfg = ServerConfig()
         cfg.hbm_budget_bytes = int(budget_bytes)
         cfg.max_batch_views = int(max_batch_views)
@@ -757,6 +760,10 @@ class Server:
         cfg.tuning = tuning if tuning is not None else default_tuning(dev)
         cfg.batch_window_ms = float(window_ms)
         cfg.simulate_mrays_per_s = float(simulate_mrays_per_s)
+        cfg.persistent_kernel = 1 if persistent else 0
+        cfg.persistent_max_requests = int(epoch_max_requests)
+        cfg.epoch_idle_ms = float(epoch_idle_ms)
+        cfg.persistent_spin_limit = int(spin_limit)
         self.h = C.c_void_p()
         check(lib().kdgrt_server_create(dev.h if dev is not None else None, C.byref(cfg), C.byref(self.h)))
 

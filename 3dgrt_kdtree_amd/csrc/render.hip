@@ -20,6 +20,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -731,6 +732,218 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 4 
             }
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent serving kernel: one launch per epoch takes view requests while
+// it runs.
+// ---------------------------------------------------------------------------
+// The host appends request records to a table in pinned host memory and
+// publishes them with a release store of the request count.  Waves claim
+// 8x8-pixel tiles from one counter in device memory, in request order.  A
+// claimed tile may not be published yet: the wave that holds the first
+// unpublished tile polls the host table, copies new records into device
+// memory and then advances the device's published-tile mark; every other
+// wave waits on that mark in device memory.  The wave that finishes a
+// request's last tile stores the request's sequence number into the host's
+// done table.  The host stops an epoch by storing 1 to `stop` after its last
+// publication; the polling wave turns it into a device stop flag and every
+// wave exits when it holds no published tile.  Every wait is bounded
+// (spin_limit polls): a wave that runs out of polls exits too, and the host
+// publishes whatever an early-ended epoch left unfinished again.  Records are
+// written once per epoch (no reuse), so a record below the published mark
+// never changes while a wave reads it.
+//
+// Memory ordering without cache maintenance on the tile path: acquire and
+// release operations at agent or system scope write back or invalidate a
+// whole L2 on this GPU, which would evict every wave's scene data at each
+// tile.  Instead every shared word goes through scope-coherent accesses --
+// relaxed atomic loads and stores at agent scope for the device tables, at
+// system scope for the host tables and the output pixels, which write
+// through to memory -- and a wave waits for its own accesses to complete
+// (workgroup-scope fence, a counter wait only) before the access that
+// publishes them: the record copies before the device mark, a tile's pixels
+// before its count.  A wave issues its loads in program order and a record
+// is read only after the mark that covers it was seen, so it reads the
+// completed record.
+struct ServeRec { // one view request, 112 B
+    uint32_t tile_begin, tile_count, scene, seq;
+    int32_t W, H, tiles_x, tiles_y;
+    int32_t groups_x, pad0;
+    uint64_t out_off; // byte offset into the output arena
+    CamInfo cam;
+    uint32_t pad1[2];
+};
+static_assert(sizeof(ServeRec) == 112, "ServeRec layout");
+constexpr uint32_t kRecWords = sizeof(ServeRec) / 4;
+
+struct ServeHostCtl { // pinned host memory, written by the host
+    uint32_t published_req;
+    uint32_t stop;
+    uint32_t pad[14];
+};
+
+struct ServeDevCtl { // device memory
+    uint32_t next_tile;
+    uint32_t mirror_tiles; // tiles of the records copied to the device
+    uint32_t mirror_req;   // records copied to the device
+    uint32_t stop;
+    uint32_t pad[12];
+};
+
+struct ServeArgs {
+    ServeHostCtl* hctl;
+    ServeRec* hrec;
+    uint32_t* hdone;        // host: sequence + 1 of each finished request
+    ServeDevCtl* dctl;
+    ServeRec* drec;
+    uint32_t* done_tiles;   // device: finished tiles per request
+    const DevScene* scenes;
+    int n_scenes;
+    char* arena;
+    int depth;
+    int max_req;
+    uint32_t spin_limit;
+};
+
+__device__ __forceinline__ uint32_t wave_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
+    return __hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t ld_system(const uint32_t* p) {
+    return __hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_system(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+// This wave's earlier memory accesses have completed (a counter wait; no
+// cache write-back or invalidation at this scope).
+__device__ __forceinline__ void wave_accesses_done() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup"); }
+
+// The wave that holds the first unpublished tile: copy newly published
+// records into device memory and advance the device marks, or turn the
+// host's stop into the device stop.  The host stores stop only after its
+// last publication, so stop is read first.
+__device__ void serve_poll_host(const ServeArgs& a, int lane) {
+    uint32_t st = 0, pr = 0;
+    if (lane == 0) {
+        st = ld_system(&a.hctl->stop);
+        pr = ld_system(&a.hctl->published_req);
+    }
+    st = wave_u32(st);
+    pr = wave_u32(pr);
+    if (pr > (uint32_t)a.max_req) pr = (uint32_t)a.max_req; // malformed count: never read past the table
+    const uint32_t mr = wave_u32(ld_agent(&a.dctl->mirror_req));
+    if (pr > mr) {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(a.hrec + mr);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(a.drec + mr);
+        const uint32_t n = (pr - mr) * kRecWords;
+        for (uint32_t i = (uint32_t)lane; i < n; i += 64) st_agent(dst + i, ld_system(src + i));
+        uint32_t end = 0;
+        if (lane == 0) {
+            const ServeRec* last = a.hrec + pr - 1;
+            end = ld_system(&last->tile_begin) + ld_system(&last->tile_count);
+        }
+        wave_accesses_done();
+        if (lane == 0) {
+            st_agent(&a.dctl->mirror_req, pr);
+            wave_accesses_done();
+            st_agent(&a.dctl->mirror_tiles, end);
+        }
+    } else if (st) {
+        if (lane == 0) st_agent(&a.dctl->stop, 1u);
+    }
+}
+
+template <int K>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_serve(ServeArgs a) {
+    const int lane = threadIdx.x;
+    extern __shared__ uint2 lds_stack[];
+    uint32_t k = 0; // the request of the last tile (tiles are claimed in increasing order)
+    while (true) {
+        uint32_t t = 0;
+        if (lane == 0) t = __hip_atomic_fetch_add(&a.dctl->next_tile, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        t = wave_u32(t);
+        bool quit = false;
+        for (uint32_t spins = 0;; ++spins) {
+            uint32_t mt = 0, stop = 0;
+            if (lane == 0) {
+                mt = ld_agent(&a.dctl->mirror_tiles);
+                stop = ld_agent(&a.dctl->stop);
+            }
+            mt = wave_u32(mt);
+            stop = wave_u32(stop);
+            if (t < mt) break;
+            if (stop || spins >= a.spin_limit) {
+                quit = true;
+                break;
+            }
+            if (t == mt) {
+                serve_poll_host(a, lane);
+                __builtin_amdgcn_s_sleep(2);
+            } else {
+                // waiting behind the polling wave: back off up to ~30 us
+                const uint32_t naps = spins < 4 ? 1u : (spins < 64 ? 4u : 16u);
+                for (uint32_t z = 0; z < naps; ++z) __builtin_amdgcn_s_sleep(127);
+            }
+        }
+        if (quit) break;
+        // the request of tile t: records below the device mark are complete
+        while (k < (uint32_t)a.max_req) {
+            const uint32_t* w = reinterpret_cast<const uint32_t*>(a.drec + k);
+            const uint32_t b = wave_u32(ld_agent(w)), c = wave_u32(ld_agent(w + 1));
+            if (t < b + c) break;
+            ++k;
+        }
+        if (k >= (uint32_t)a.max_req) break; // malformed table: never read past it
+        ServeRec r;
+        {
+            const uint32_t* src = reinterpret_cast<const uint32_t*>(a.drec + k);
+            const uint32_t mine = lane < (int)kRecWords ? ld_agent(src + lane) : 0u;
+            uint32_t* dst = reinterpret_cast<uint32_t*>(&r);
+#pragma unroll
+            for (int w = 0; w < (int)kRecWords; ++w) dst[w] = __builtin_amdgcn_readlane(mine, w);
+        }
+        const uint32_t local = t - r.tile_begin;
+        const uint32_t grp = local / 64u, gl = local % 64u;
+        const int tx = (int)((grp % (uint32_t)r.groups_x) * 8u + gl % 8u);
+        const int ty = (int)((grp / (uint32_t)r.groups_x) * 8u + gl / 8u);
+        if (tx < r.tiles_x && ty < r.tiles_y && r.scene < (uint32_t)a.n_scenes) {
+            const DevScene sc = a.scenes[r.scene];
+            const int x = tx * 8 + (lane & 7), y = ty * 8 + (lane >> 3);
+            if (x < r.W && y < r.H) {
+                float o[3], d[3], rgb[3], A;
+                make_ray(r.cam, x, y, o, d);
+                trace_ray<K, false>(sc, o, d, a.depth, lds_stack, lane, rgb, A, nullptr);
+                uint32_t* px = reinterpret_cast<uint32_t*>(a.arena + r.out_off) + 4 * ((size_t)(r.H - y - 1) * r.W + x);
+                st_system(px + 0, __float_as_uint(rgb[0]));
+                st_system(px + 1, __float_as_uint(rgb[1]));
+                st_system(px + 2, __float_as_uint(rgb[2]));
+                st_system(px + 3, __float_as_uint(1.0f - A));
+            }
+        }
+        // the tile's pixels are written through before it counts as finished
+        wave_accesses_done();
+        if (lane == 0) {
+            const uint32_t c = __hip_atomic_fetch_add(&a.done_tiles[k], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (c + 1 == r.tile_count) __hip_atomic_store(&a.hdone[k], r.seq + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
+}
+
+__global__ void k_serve_reset(ServeDevCtl* c, uint32_t* done_tiles, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        c->next_tile = 0;
+        c->mirror_tiles = 0;
+        c->mirror_req = 0;
+        c->stop = 0;
+    }
+    for (int j = i; j < n; j += gridDim.x * blockDim.x) done_tiles[j] = 0;
 }
 
 typedef void (*BatchKernel)(BatchArgs);
@@ -2171,6 +2384,169 @@ void slot_destroy(BatchSlot* sl) {
     if (sl->d_items) hipFree(sl->d_items);
     if (sl->d_counters) hipFree(sl->d_counters);
     delete sl;
+}
+
+// Persistent serving queue (k_serve): pinned host table (control block,
+// request records, done flags) and device table (control block, finished-
+// tile counters, record copies, scene table), allocated once.
+struct ServeQueue {
+    kdgrt_context* ctx = nullptr;
+    int max_req = 0;
+    void* hblock = nullptr;
+    ServeHostCtl* hctl = nullptr;
+    ServeRec* hrec = nullptr;
+    uint32_t* hdone = nullptr;
+    void* dblock = nullptr;
+    ServeArgs args{};
+    int grid = 0;
+    size_t lds = 0;
+    uint32_t seq = 0;
+    uint64_t tiles = 0;
+    bool running = false;
+    hipEvent_t ev = nullptr;
+    hipStream_t stream = nullptr;
+};
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+kdgrt_status serve_queue_create(kdgrt_context* ctx, int max_req, const kdgrt_scene* const* scenes, int n_scenes,
+                                void* arena, hipStream_t stream, const kdgrt_tuning& t, uint32_t spin_limit,
+                                ServeQueue** out) {
+    if (!ctx || !out || max_req < 1 || max_req > (1 << 20) || n_scenes < 1 || !arena || spin_limit < 1 ||
+        t.stack_depth < 1 || t.stack_depth > 32 || t.waves_per_cu < 1)
+        return set_error(KDGRT_ERR_INVALID_ARGUMENT, "serve_queue_create: bad arguments");
+    auto* q = new ServeQueue();
+    q->ctx = ctx;
+    q->max_req = max_req;
+    q->stream = stream;
+    const size_t h_ctl = align256(sizeof(ServeHostCtl)), h_rec = align256(sizeof(ServeRec) * (size_t)max_req),
+                 h_done = align256(4ull * max_req);
+    const size_t d_ctl = align256(sizeof(ServeDevCtl)), d_done = align256(4ull * max_req), d_rec = h_rec,
+                 d_sc = align256(sizeof(DevScene) * (size_t)n_scenes);
+    hipError_t e = hipSetDevice(ctx->device);
+    // coherent (fine-grained) pinned memory: the kernel reads the host's
+    // stores while it runs, and the host reads the kernel's done flags
+    if (e == hipSuccess) e = hipHostMalloc(&q->hblock, h_ctl + h_rec + h_done, hipHostMallocCoherent | hipHostMallocMapped);
+    if (e == hipSuccess) e = hipMalloc(&q->dblock, d_ctl + d_done + d_rec + d_sc);
+    void* hdev = nullptr;
+    if (e == hipSuccess) e = hipHostGetDevicePointer(&hdev, q->hblock, 0);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&q->ev, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        if (q->hblock) hipHostFree(q->hblock);
+        if (q->dblock) hipFree(q->dblock);
+        delete q;
+        return set_error(e == hipErrorOutOfMemory ? KDGRT_ERR_OUT_OF_MEMORY : KDGRT_ERR_DEVICE,
+                         std::string("serve_queue_create: ") + hipGetErrorString(e));
+    }
+    std::memset(q->hblock, 0, h_ctl + h_rec + h_done);
+    q->hctl = (ServeHostCtl*)q->hblock;
+    q->hrec = (ServeRec*)((char*)q->hblock + h_ctl);
+    q->hdone = (uint32_t*)((char*)q->hblock + h_ctl + h_rec);
+    std::vector<DevScene> sc(n_scenes);
+    for (int i = 0; i < n_scenes; ++i) sc[i] = scenes[i]->dev;
+    char* db = (char*)q->dblock;
+    ServeArgs& a = q->args;
+    a.hctl = (ServeHostCtl*)hdev;
+    a.hrec = (ServeRec*)((char*)hdev + h_ctl);
+    a.hdone = (uint32_t*)((char*)hdev + h_ctl + h_rec);
+    a.dctl = (ServeDevCtl*)db;
+    a.done_tiles = (uint32_t*)(db + d_ctl);
+    a.drec = (ServeRec*)(db + d_ctl + d_done);
+    a.scenes = (const DevScene*)(db + d_ctl + d_done + d_rec);
+    a.n_scenes = n_scenes;
+    a.arena = (char*)arena;
+    a.depth = t.stack_depth;
+    a.max_req = max_req;
+    a.spin_limit = spin_limit;
+    e = hipMemcpy((void*)a.scenes, sc.data(), sizeof(DevScene) * n_scenes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        serve_queue_destroy(q);
+        return set_error(KDGRT_ERR_DEVICE, std::string("serve_queue_create: ") + hipGetErrorString(e));
+    }
+    const int n_cu = ctx->prop.multiProcessorCount > 0 ? ctx->prop.multiProcessorCount : 256;
+    q->grid = std::max(1, t.waves_per_cu * n_cu);
+    q->lds = stack_lds_bytes(t.stack_depth, 64);
+    if (q->lds > ctx->prop.sharedMemPerBlock) {
+        serve_queue_destroy(q);
+        return set_error(KDGRT_ERR_INVALID_ARGUMENT, "serve_queue_create: stack depth exceeds the LDS");
+    }
+    *out = q;
+    return KDGRT_OK;
+}
+
+void serve_queue_destroy(ServeQueue* q) {
+    if (!q) return;
+    hipSetDevice(q->ctx->device);
+    if (q->running) {
+        serve_epoch_stop(q);
+        hipEventSynchronize(q->ev);
+    }
+    if (q->ev) hipEventDestroy(q->ev);
+    if (q->dblock) hipFree(q->dblock);
+    if (q->hblock) hipHostFree(q->hblock);
+    delete q;
+}
+
+kdgrt_status serve_epoch_start(ServeQueue* q) {
+    if (q->running) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "serve_epoch_start: an epoch is running");
+    // no kernel reads the host table between epochs
+    q->hctl->published_req = 0;
+    q->hctl->stop = 0;
+    std::memset(q->hdone, 0, 4ull * q->max_req);
+    std::atomic_thread_fence(std::memory_order_seq_cst);
+    q->seq = 0;
+    q->tiles = 0;
+    const int rb = (int)std::min<int64_t>(1024, (q->max_req + 255) / 256);
+    hipLaunchKernelGGL(k_serve_reset, dim3(rb), dim3(256), 0, q->stream, q->args.dctl, q->args.done_tiles, q->max_req);
+    hipLaunchKernelGGL(k_serve<kViewK>, dim3(q->grid), dim3(64), q->lds, q->stream, q->args);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(q->ev, q->stream));
+    q->running = true;
+    return KDGRT_OK;
+}
+
+void serve_epoch_stop(ServeQueue* q) { __atomic_store_n(&q->hctl->stop, 1u, __ATOMIC_RELEASE); }
+
+bool serve_epoch_finished(ServeQueue* q, kdgrt_status* status) {
+    if (!q->running) return true;
+    const hipError_t e = hipEventQuery(q->ev);
+    if (e == hipErrorNotReady) return false;
+    if (status) *status = e == hipSuccess ? KDGRT_OK : KDGRT_ERR_DEVICE;
+    q->running = false;
+    return true;
+}
+
+bool serve_room(const ServeQueue* q, int W, int H) {
+    const uint64_t tx = (W + 7) / 8, ty = (H + 7) / 8;
+    const uint64_t tiles = ((tx + 7) / 8) * ((ty + 7) / 8) * 64;
+    return q->running && q->seq < (uint32_t)q->max_req && q->tiles + tiles < (1ull << 31);
+}
+
+kdgrt_status serve_publish(ServeQueue* q, int scene_index, const kdgrt_camera_info& cam, int W, int H,
+                           uint64_t out_off, uint32_t* seq) {
+    if (!serve_room(q, W, H) || scene_index < 0 || scene_index >= q->args.n_scenes || W < 1 || H < 1)
+        return set_error(KDGRT_ERR_CAPACITY, "serve_publish: no room in the epoch");
+    ServeRec& r = q->hrec[q->seq];
+    r.tiles_x = (W + 7) / 8;
+    r.tiles_y = (H + 7) / 8;
+    r.groups_x = (r.tiles_x + 7) / 8;
+    r.tile_begin = (uint32_t)q->tiles;
+    r.tile_count = (uint32_t)r.groups_x * (uint32_t)((r.tiles_y + 7) / 8) * 64u;
+    r.scene = (uint32_t)scene_index;
+    r.seq = q->seq;
+    r.W = W;
+    r.H = H;
+    r.pad0 = 0;
+    r.out_off = out_off;
+    std::memcpy(&r.cam, &cam, sizeof(CamInfo));
+    q->tiles += r.tile_count;
+    *seq = q->seq++;
+    __atomic_store_n(&q->hctl->published_req, q->seq, __ATOMIC_RELEASE);
+    return KDGRT_OK;
+}
+
+bool serve_done(const ServeQueue* q, uint32_t seq) {
+    return __atomic_load_n(&q->hdone[seq], __ATOMIC_ACQUIRE) == seq + 1u;
 }
 
 kdgrt_status slot_launch(BatchSlot* sl, const kdgrt_view* views, int n, const kdgrt_tuning& t, hipStream_t s,

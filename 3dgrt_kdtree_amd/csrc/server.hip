@@ -19,6 +19,13 @@
 //  * A batch that holds a single view runs the one-view kernel (k_view with
 //    its cone pre-test) on the slot's stream; larger batches run the
 //    persistent kernel.
+//  * persistent_kernel = 1: one long-running persistent kernel (k_serve) per
+//    epoch takes every request; the scheduler publishes each admitted request
+//    into the kernel's queue while the kernel runs, so requests join the
+//    running kernel's tile stream as they arrive (continuous batching into
+//    the persistent kernel).  An idle epoch is stopped after epoch_idle_ms
+//    and the next request starts a new one; requests an epoch leaves
+//    unfinished (its kernel ended early) are published again in the next.
 //  * plain_launches = 1 is the comparison mode: each request is one plain
 //    one-view launch (k_view), run one at a time, as the reference renders.
 //  * With ctx == NULL and simulate_mrays_per_s > 0 the same scheduler runs
@@ -149,6 +156,14 @@ struct kdgrt_server {
     hipStream_t copy_stream = nullptr;
     std::vector<hipStream_t> copy_pool; // result-copy streams of kdgrt_server_wait (copy_stream among them)
     double sim_gpu_free = 0;
+
+    // persistent mode
+    kdgrt::ServeQueue* sq = nullptr;
+    hipStream_t serve_stream = nullptr;
+    std::vector<std::pair<Req*, uint32_t>> serving; // published in the running epoch: (request, sequence)
+    std::map<int32_t, int> scene_index;             // scene id -> index in the kernel's scene table
+    bool epoch_running = false, epoch_stopping = false;
+    double last_busy = 0;
 
     std::thread worker;
     bool started = false, stopping = false;
@@ -298,6 +313,123 @@ bool launch_one(kdgrt_server* s) {
     return true;
 }
 
+// Persistent mode: completions, epoch ends, admission into the running
+// kernel, idle stop.  Caller holds the lock.  Returns whether anything changed.
+bool persist_step(kdgrt_server* s) {
+    bool changed = false;
+    const double t = s->now();
+    for (size_t i = 0; i < s->serving.size();) {
+        Req* r = s->serving[i].first;
+        if (!kdgrt::serve_done(s->sq, s->serving[i].second)) {
+            ++i;
+            continue;
+        }
+        r->done = t;
+        r->state = kDone;
+        r->status = KDGRT_OK;
+        s->st.completed++;
+        s->st.rays += (uint64_t)r->r.width * r->r.height;
+        s->serving.erase(s->serving.begin() + i);
+        changed = true;
+    }
+    if (s->epoch_running) {
+        kdgrt_status est = KDGRT_OK;
+        if (kdgrt::serve_epoch_finished(s->sq, &est)) {
+            // the kernel exited: stopped by us, or early (its waits ran out);
+            // what it left unfinished runs again in the next epoch, in order
+            s->epoch_running = s->epoch_stopping = false;
+            for (auto it = s->serving.rbegin(); it != s->serving.rend(); ++it) {
+                Req* r = it->first;
+                if (est != KDGRT_OK) {
+                    r->state = kFailed;
+                    r->status = est;
+                    r->done = t;
+                } else {
+                    r->state = kPending;
+                    s->pending.push_front(r);
+                }
+            }
+            s->serving.clear();
+            if (--s->busy_count == 0) s->st.busy_s += t - s->busy_since;
+            changed = true;
+        }
+    }
+    // requests whose deadline passed before they were published
+    for (auto it = s->pending.begin(); it != s->pending.end();) {
+        Req* r = *it;
+        if (r->r.deadline_s > 0 && t > r->r.deadline_s) {
+            r->state = kCancelled;
+            r->status = KDGRT_ERR_CANCELLED;
+            r->done = t;
+            s->st.cancelled++;
+            it = s->pending.erase(it);
+            changed = true;
+        } else {
+            ++it;
+        }
+    }
+    while (!s->pending.empty() && !s->epoch_stopping) {
+        Req* r = s->pending.front();
+        if (!r->has_mem) {
+            uint64_t off = 0;
+            if (!s->arena.alloc(r->bytes, &off)) break; // arrival order: no overtaking
+            r->off = off;
+            r->has_mem = true;
+        }
+        if (!s->epoch_running) {
+            const kdgrt_status st = kdgrt::serve_epoch_start(s->sq);
+            if (st != KDGRT_OK) {
+                r->state = kFailed;
+                r->status = st;
+                r->done = t;
+                s->pending.pop_front();
+                changed = true;
+                continue;
+            }
+            s->epoch_running = true;
+            s->st.batches++;
+            if (s->busy_count++ == 0) s->busy_since = t;
+        }
+        if (!kdgrt::serve_room(s->sq, r->r.width, r->r.height)) {
+            kdgrt::serve_epoch_stop(s->sq); // the epoch's tables are full: drain, then a new epoch
+            s->epoch_stopping = true;
+            break;
+        }
+        uint32_t seq = 0;
+        const kdgrt_status st = kdgrt::serve_publish(s->sq, s->scene_index[r->r.scene_id], r->r.cam, r->r.width,
+                                                     r->r.height, r->off, &seq);
+        if (st != KDGRT_OK) break;
+        s->pending.pop_front();
+        r->state = kAdmitted;
+        r->admitted = r->launched = t;
+        r->batch_id = (uint32_t)s->st.batches;
+        s->serving.push_back({r, seq});
+        r->batch_views = (uint32_t)s->serving.size(); // requests in the running kernel when it joined
+        s->st.bytes_peak = std::max<uint64_t>(s->st.bytes_peak, s->scene_bytes + s->arena.used());
+        changed = true;
+    }
+    if (!s->serving.empty() || !s->pending.empty()) s->last_busy = t;
+    if (s->epoch_running && !s->epoch_stopping && s->serving.empty() && s->pending.empty() &&
+        (s->stopping || t - s->last_busy > std::max(0.0, s->cfg.epoch_idle_ms) * 1e-3)) {
+        kdgrt::serve_epoch_stop(s->sq);
+        s->epoch_stopping = true;
+    }
+    return changed;
+}
+
+void worker_main_persistent(kdgrt_server* s) {
+    hipSetDevice(kdgrt::context_device(s->ctx));
+    std::unique_lock<std::mutex> lk(s->mu);
+    while (true) {
+        if (persist_step(s)) s->cv_done.notify_all();
+        if (s->stopping && s->pending.empty() && s->serving.empty() && !s->epoch_running) break;
+        // completions are flags in host memory: poll them briskly while work runs
+        const double wait_s = (!s->serving.empty() || s->epoch_stopping) ? 10e-6 : (s->epoch_running ? 200e-6 : -1.0);
+        if (wait_s > 0) s->cv_work.wait_for(lk, std::chrono::duration<double>(wait_s));
+        else s->cv_work.wait(lk);
+    }
+}
+
 void worker_main(kdgrt_server* s) {
     if (!s->sim) hipSetDevice(kdgrt::context_device(s->ctx));
     std::unique_lock<std::mutex> lk(s->mu);
@@ -444,9 +576,24 @@ kdgrt_status kdgrt_server_start(kdgrt_server* s) {
     }
     s->slot_busy.assign(n, false);
     s->arena.reset(arena);
+    if (s->cfg.persistent_kernel && !s->sim) {
+        std::vector<const kdgrt_scene*> table;
+        for (auto& kv : s->scenes) {
+            s->scene_index[kv.first] = (int)table.size();
+            table.push_back(kv.second.first);
+        }
+        if (table.empty()) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "server_start: no scenes");
+        if (hipStreamCreateWithFlags(&s->serve_stream, hipStreamNonBlocking) != hipSuccess)
+            return set_error(KDGRT_ERR_DEVICE, "server_start: stream");
+        const int max_req = s->cfg.persistent_max_requests > 0 ? s->cfg.persistent_max_requests : 4096;
+        const uint32_t spins = s->cfg.persistent_spin_limit > 0 ? s->cfg.persistent_spin_limit : (1u << 22);
+        kdgrt_status k = kdgrt::serve_queue_create(s->ctx, max_req, table.data(), (int)table.size(), s->d_arena,
+                                                   s->serve_stream, s->cfg.tuning, spins, &s->sq);
+        if (k != KDGRT_OK) return k;
+    }
     s->started = true;
     s->stopping = false;
-    s->worker = std::thread(worker_main, s);
+    s->worker = std::thread(s->sq ? worker_main_persistent : worker_main, s);
     return KDGRT_OK;
 }
 
@@ -590,6 +737,8 @@ kdgrt_status kdgrt_server_destroy(kdgrt_server* s) {
         for (auto e : s->events) hipEventDestroy(e);
         for (auto st : s->streams) hipStreamDestroy(st);
         for (auto cs : s->copy_pool) hipStreamDestroy(cs);
+        if (s->sq) kdgrt::serve_queue_destroy(s->sq);
+        if (s->serve_stream) hipStreamDestroy(s->serve_stream);
         if (s->d_arena) hipFree(s->d_arena);
         for (auto& kv : s->scenes)
             if (kv.second.first && s->scene_owned[kv.first]) kdgrt_scene_destroy(kv.second.first);

@@ -189,6 +189,7 @@ def run_poisson(srv, ids, aabbs, lam, duration_s, seed):
             "mrays_s": round(rays / span / 1e6, 1), "p50_ms": round(float(np.percentile(lat, 50)), 3),
             "p99_ms": round(float(np.percentile(lat, 99)), 3), "max_ms": round(float(lat.max()), 3),
             "mean_views_per_launch": round(float(views.mean()), 2),
+            "max_views_per_launch": int(views.max()),
             "failed": int(sum(1 for x in infos if x.status != 0))}
 
 
@@ -286,17 +287,19 @@ def multi_scene_batch(dev, scenes, aabbs, counts, tuning, n_views=256, n_rays=4_
 
 def serve_bench(dev, scenes, aabbs, counts, tuning, rates=((200, 4.0), (1000, 2.0)), window_ms=0.0):
     """Serving: the resident scenes (HBM budget = resident bytes / 0.8),
-    Poisson arrivals, one view per request; continuous batching against
-    plain one-launch-per-request."""
+    Poisson arrivals, one view per request, in three modes: continuous
+    batching into one running persistent kernel ("persistent"), overlapping
+    batched launches on three streams ("batched"), and plain one-launch-per-
+    request one at a time ("plain", the reference's way)."""
     resident = sum(sc.device_bytes() for sc in scenes)
     budget = int(resident / 0.8)
     log(f"serving: {len(scenes)} scenes, {int(counts.sum())} Gaussians, resident {resident / 1e9:.2f} GB, "
         f"budget {budget / 1e9:.2f} GB")
     out = {"scenes": len(scenes), "gaussians": [int(c) for c in counts], "resident_bytes": resident,
            "hbm_budget_bytes": budget, "batch_window_ms": window_ms, "max_batch_views": 32, "inflight_batches": 3}
-    for mode in ("batched", "plain"):
+    for mode in ("persistent", "batched", "plain"):
         srv = K.Server(dev, budget, max_batch_views=32, max_inflight=3, plain=(mode == "plain"), tuning=tuning,
-                       window_ms=window_ms)
+                       window_ms=window_ms, persistent=(mode == "persistent"))
         ids = [srv.add_resident_scene(sc) for sc in scenes]
         srv.start()
         for lam, dur in rates:
@@ -307,6 +310,7 @@ def serve_bench(dev, scenes, aabbs, counts, tuning, rates=((200, 4.0), (1000, 2.
     for lam, _ in rates:
         e = out[f"poisson_{lam}"]
         e["served_vs_plain_mrays"] = round(e["batched"]["mrays_s"] / e["plain"]["mrays_s"], 3)
+        e["persistent_vs_plain_mrays"] = round(e["persistent"]["mrays_s"] / e["plain"]["mrays_s"], 3)
     return out
 
 

@@ -418,6 +418,14 @@ typedef struct kdgrt_server_config {
     kdgrt_tuning tuning;
     double batch_window_ms;      /* while the device is busy, wait up to this long to fill a batch */
     double simulate_mrays_per_s; /* > 0 with ctx == NULL: no device, batches take rays / rate (tests) */
+    /* 1 = continuous batching into one running persistent kernel per epoch:
+     * admitted requests are published into the kernel's queue while it runs
+     * (coherent host memory, polled by the kernel); batch and window fields
+     * then do not apply. */
+    int32_t persistent_kernel;
+    int32_t persistent_max_requests; /* requests per epoch (0: 4096); a full epoch drains and a new one starts */
+    double epoch_idle_ms;            /* an idle epoch's kernel is stopped after this long */
+    uint32_t persistent_spin_limit;  /* polls a kernel wave makes before it gives up waiting (0: 4M) */
 } kdgrt_server_config;
 
 typedef struct kdgrt_request {
