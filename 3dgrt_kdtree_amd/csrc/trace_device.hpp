@@ -160,7 +160,8 @@ __device__ __forceinline__ void sh_basis(const float d[3], float s[16]) {
 // of degree 1 (minus, plus, minus), +0.5, clamped below at 0.  The device SH
 // record is channel-major: channel c occupies float4s 4c..4c+3 as
 // (f_dc[c], f_rest[15c+0 .. 15c+14]), so one channel is loaded at a time.
-__device__ __forceinline__ void sh_radiance(const float4* __restrict__ sh, const float s[16], float rad[3]) {
+template <class SH>
+__device__ __forceinline__ void sh_radiance(const SH& sh, const float s[16], float rad[3]) {
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
         float c[16];
@@ -615,12 +616,25 @@ __device__ __forceinline__ void leaf(RayState& st, const DevScene& sc, RayDbg* d
                 dbg->cnt[6]++;
                 if (dbg->t_eval) dbg->t_eval[kid[i]] = 1;
             }
+            // View kernels (cone pre-test) request the 192-B SH record before
+            // the response is formed, so its latency hides behind kd_expf
+            // and the basis; only samples that the skip rule drops (alpha <
+            // 1/255; grayDist < 8 keeps the response above 0.0113) waste the
+            // load.  The other kernels hold more state and would spill, so
+            // they request it after the skip rule.
+            constexpr bool kEarlySH = CONE && !HYB;
+            const uint32_t gid = kid[i];
+            const float4* shp = sc.shade + 12 * (size_t)gid;
+            float4 shq[kEarlySH ? 12 : 1];
+            if (kEarlySH) {
+#pragma unroll
+                for (int j = 0; j < (kEarlySH ? 12 : 1); ++j) shq[j] = shp[j];
+            }
             // particleResponse<4> and the skip rule (cudaRenderer.cu:1655-1671)
             const float gd = kg[i];
             const float gres = kd_expf(-0.0555555555556f * gd * gd);
             const float alpha = fminf(0.99f, ko[i] * gres);
             if (gres < 0.0113f || alpha < 1.0f / 255.0f) continue;
-            const uint32_t gid = kid[i];
             float c[3];
             // The basis depends only on the ray direction; it is formed here,
             // per blended sample, rather than held in 16 registers or 4 KB of
@@ -630,7 +644,8 @@ __device__ __forceinline__ void leaf(RayState& st, const DevScene& sc, RayDbg* d
             const float dv[3] = {bx, by, bz};
             float shs[16];
             sh_basis(dv, shs);
-            sh_radiance(sc.shade + 12 * (size_t)gid, shs, c);
+            if (kEarlySH) sh_radiance(shq, shs, c);
+            else sh_radiance(shp, shs, c); // channel by channel from memory
             const float w = 1.0f - st.A;
             st.rgb[0] = st.rgb[0] + (c[0] * alpha) * w;
             st.rgb[1] = st.rgb[1] + (c[1] * alpha) * w;
