@@ -346,6 +346,39 @@ def cam_info_bytes(ci):
     return bytes(ci)
 
 
+# ------------------------------------------------------- viewer camera controls
+KEY_W, KEY_S, KEY_A, KEY_D, KEY_Q, KEY_E = 1, 2, 4, 8, 16, 32
+# the reference viewer's defaults (DefineConst.h:16-17, updateOrbitCamera)
+CAM_MOVE_SPEED, CAM_ROT_SPEED, ORBIT_SPEED = 0.5, 0.1, 0.8
+ORBIT_CENTRE, ORBIT_AXIS = (0.393, 1.093, 0.698), (0.0138, 0.9038, 0.4278)
+
+
+def camera_copy(cam):
+    c = Camera()
+    C.memmove(C.byref(c), C.byref(cam), C.sizeof(Camera))
+    return c
+
+
+def camera_move(cam, keys, speed=CAM_MOVE_SPEED, dt=1.0 / 60.0):
+    """Held-key move (KEY_* bits) in place; returns whether a key was held."""
+    moved = C.c_int()
+    check(lib().kdgrt_camera_move(C.byref(cam), C.c_uint32(keys), C.c_float(speed), C.c_float(dt),
+                                  C.byref(moved)))
+    return bool(moved.value)
+
+
+def camera_orbit(cam, angle, centre=ORBIT_CENTRE, axis=ORBIT_AXIS):
+    """Orbit about the line (centre, axis) by `angle` radians, in place."""
+    ce, ax = (C.c_float * 3)(*centre), (C.c_float * 3)(*axis)
+    check(lib().kdgrt_camera_orbit(C.byref(cam), ce, ax, C.c_float(angle)))
+
+
+def camera_drag(cam, delx, dely, rot_speed=CAM_ROT_SPEED, zoom=False):
+    """Mouse drag: yaw/pitch (or fovy zoom with zoom=True), in place."""
+    check(lib().kdgrt_camera_drag(C.byref(cam), C.c_int(delx), C.c_int(dely), C.c_float(rot_speed),
+                                  C.c_int(1 if zoom else 0)))
+
+
 # ------------------------------------------------------------------ device side
 class Device:
     """A kdgrt context on one HIP device plus helpers for device buffers."""
@@ -818,6 +851,64 @@ class Server:
         if self.h:
             lib().kdgrt_server_stop(self.h)
             lib().kdgrt_server_destroy(self.h)
+            self.h = None
+
+
+FRAME_RGB_F32, FRAME_RGBA8, FRAME_RGB8 = 0, 1, 2
+
+
+class ViewerFrame(C.Structure):
+    _fields_ = [("frame_id", C.c_uint64), ("pixels", C.c_void_p), ("bytes", C.c_uint64), ("width", C.c_int),
+                ("height", C.c_int), ("format", C.c_int), ("render_ms", C.c_float), ("fps", C.c_float),
+                ("camera", Camera)]
+
+
+class ViewerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("delivered", C.c_uint64), ("mean_render_ms", C.c_double),
+                ("window_avg_fps", C.c_double), ("window_frames", C.c_uint64), ("wall_fps", C.c_double)]
+
+
+class Viewer:
+    """Frame loop of the interactive viewer (kdgrt_viewer_*): frames rendered
+    into a ring of slots and copied to pinned host memory while the next
+    frame renders; acquire() returns them in submission order."""
+
+    def __init__(self, dev, scene, width, height, fmt=FRAME_RGBA8, slots=3, stack_depth=12, window=(500, 1000)):
+        self.dev, self.width, self.height, self.format = dev, width, height, fmt
+        self.h = C.c_void_p()
+        check(lib().kdgrt_viewer_create(dev.h, scene.h, C.c_int(width), C.c_int(height), C.c_int(fmt),
+                                        C.c_int(slots), C.c_int(stack_depth), C.c_int(window[0]), C.c_int(window[1]),
+                                        C.byref(self.h)))
+
+    def submit(self, cam, timeout_ms=-1):
+        fid = C.c_uint64()
+        check(lib().kdgrt_viewer_submit(self.h, C.byref(cam), C.c_int(timeout_ms), C.byref(fid)))
+        return fid.value
+
+    def acquire(self, timeout_ms=-1, copy=True):
+        """(frame info, pixels): pixels is a copy (copy=True) or a view of the
+        slot's pinned memory, valid until release(frame_id)."""
+        f = ViewerFrame()
+        check(lib().kdgrt_viewer_acquire(self.h, C.c_int(timeout_ms), C.byref(f)))
+        if f.format == FRAME_RGB_F32:
+            dt, shape = np.float32, (f.height, f.width, 3)
+        else:
+            dt, shape = np.uint8, (f.height, f.width, 4 if f.format == FRAME_RGBA8 else 3)
+        buf = (C.c_char * f.bytes).from_address(f.pixels)
+        px = np.frombuffer(buf, dtype=dt).reshape(shape)
+        return f, (px.copy() if copy else px)
+
+    def release(self, frame_id):
+        check(lib().kdgrt_viewer_release(self.h, C.c_uint64(frame_id)))
+
+    def stats(self):
+        st = ViewerStats()
+        check(lib().kdgrt_viewer_stats_get(self.h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in ViewerStats._fields_}
+
+    def close(self):
+        if self.h:
+            lib().kdgrt_viewer_destroy(self.h)
             self.h = None
 
 

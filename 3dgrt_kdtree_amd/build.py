@@ -24,7 +24,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
 HOST_SOURCES = ["host_core.cpp", "sah_builder.cpp", "ingest_stats.cpp"]
-HIP_SOURCES = ["render.hip", "autotune.hip", "server.hip", "host_api.hip"]
+HIP_SOURCES = ["render.hip", "autotune.hip", "server.hip", "host_api.hip", "viewer.hip"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fvisibility=hidden", "-I" + INC, "-I" + CSRC]
 HOST_FLAGS = COMMON + ["-fopenmp", "-fno-math-errno", "-Wall", "-Wno-unknown-pragmas"]

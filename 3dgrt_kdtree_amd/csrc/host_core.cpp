@@ -337,6 +337,148 @@ kdgrt_status kdgrt_camera_make(const kdgrt_camera* cam, int width, int height, k
 } // extern "C"
 
 // ---------------------------------------------------------------------------
+// Viewer camera controls (Kd-treeConverterMain.cpp:422-480, 1193-1296,
+// 3463-3500), f32 host arithmetic as there.
+// ---------------------------------------------------------------------------
+namespace {
+
+// fMyVecNormalize (MyMathUtility.cpp:77-90): float squares summed in float,
+// the length in double; a vector shorter than 1e-8 becomes zero.
+void vec_normalize(float* v) {
+    const double len = std::sqrt((double)(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+    if (len < 0.00000001) {
+        v[0] = v[1] = v[2] = 0.0f;
+        return;
+    }
+    for (int i = 0; i < 3; ++i) v[i] = (float)((double)v[i] / len);
+}
+
+// fMyVecCrossProduct (MyMathUtility.cpp:59-63): out = a x b.
+void vec_cross(const float* a, const float* b, float* out) {
+    out[0] = (a[1] * b[2]) - (a[2] * b[1]);
+    out[1] = -((a[0] * b[2]) - (a[2] * b[0]));
+    out[2] = (a[0] * b[1]) - (a[1] * b[0]);
+}
+
+// n normalised, v = n x u, u = v x n, each normalised: the orthonormalisation
+// that follows every rotation in the reference viewer.
+void reorthonormalise(kdgrt_camera& c) {
+    vec_normalize(c.n);
+    vec_cross(c.n, c.u, c.v);
+    vec_normalize(c.v);
+    vec_cross(c.v, c.n, c.u);
+    vec_normalize(c.u);
+}
+
+// rotateVectorAroundArbitraryAxis (Kd-treeConverterMain.cpp:422-436).
+void rotate_about(float* vec, const float* axis, float cos_a, float sin_a) {
+    const float cr[3] = {axis[1] * vec[2] - axis[2] * vec[1], axis[2] * vec[0] - axis[0] * vec[2],
+                         axis[0] * vec[1] - axis[1] * vec[0]};
+    const float dot = axis[0] * vec[0] + axis[1] * vec[1] + axis[2] * vec[2];
+    for (int i = 0; i < 3; ++i) vec[i] = vec[i] * cos_a + cr[i] * sin_a + axis[i] * dot * (1.0f - cos_a);
+}
+
+// The matrix of glRotatef(angle_deg, x, y, z) as the OpenGL specification
+// defines it (axis normalised, m[r][c]); a zero axis gives the identity.
+void gl_rotation(float angle_deg, float x, float y, float z, float m[3][3]) {
+    const float mag = std::sqrt(x * x + y * y + z * z);
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) m[r][c] = r == c ? 1.0f : 0.0f;
+    if (!(mag > 0.0f)) return;
+    x /= mag;
+    y /= mag;
+    z /= mag;
+    const float a = angle_deg * (3.14159265358979323846f / 180.0f);
+    const float c = std::cos(a), s = std::sin(a), oc = 1.0f - c;
+    m[0][0] = x * x * oc + c;
+    m[0][1] = x * y * oc - z * s;
+    m[0][2] = x * z * oc + y * s;
+    m[1][0] = y * x * oc + z * s;
+    m[1][1] = y * y * oc + c;
+    m[1][2] = y * z * oc - x * s;
+    m[2][0] = x * z * oc - y * s;
+    m[2][1] = y * z * oc + x * s;
+    m[2][2] = z * z * oc + c;
+}
+
+// vec := m vec, the reference's R[0] x + R[4] y + R[8] z ... on the
+// column-major GL matrix.
+void apply(const float m[3][3], float* vec) {
+    const float t[3] = {vec[0], vec[1], vec[2]};
+    for (int r = 0; r < 3; ++r) vec[r] = m[r][0] * t[0] + m[r][1] * t[1] + m[r][2] * t[2];
+}
+
+} // namespace
+
+extern "C" {
+
+kdgrt_status kdgrt_camera_move(kdgrt_camera* cam, uint32_t keys, float speed, float dt, int* moved) {
+    if (!cam) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "camera_move: null camera");
+    const float s = speed * dt;
+    struct Step {
+        uint32_t key;
+        const float* axis;
+        float sign;
+    };
+    const Step steps[6] = {{KDGRT_KEY_W, cam->n, -1.0f}, {KDGRT_KEY_S, cam->n, 1.0f},
+                           {KDGRT_KEY_A, cam->u, -1.0f}, {KDGRT_KEY_D, cam->u, 1.0f},
+                           {KDGRT_KEY_Q, cam->v, 1.0f},  {KDGRT_KEY_E, cam->v, -1.0f}};
+    int any = 0;
+    for (const Step& st : steps) {
+        if (!(keys & st.key)) continue;
+        any = 1;
+        for (int i = 0; i < 3; ++i) {
+            const float d = st.axis[i] * s;
+            cam->pos[i] = st.sign < 0.0f ? cam->pos[i] - d : cam->pos[i] + d;
+        }
+    }
+    if (moved) *moved = any;
+    return KDGRT_OK;
+}
+
+kdgrt_status kdgrt_camera_orbit(kdgrt_camera* cam, const float centre[3], const float axis[3], float angle) {
+    if (!cam || !centre || !axis) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "camera_orbit: null argument");
+    const float c = std::cos(angle), s = std::sin(angle);
+    float p[3] = {cam->pos[0] - centre[0], cam->pos[1] - centre[1], cam->pos[2] - centre[2]};
+    rotate_about(p, axis, c, s);
+    for (int i = 0; i < 3; ++i) cam->pos[i] = p[i] + centre[i];
+    rotate_about(cam->u, axis, c, s);
+    rotate_about(cam->v, axis, c, s);
+    rotate_about(cam->n, axis, c, s);
+    reorthonormalise(*cam);
+    return KDGRT_OK;
+}
+
+kdgrt_status kdgrt_camera_drag(kdgrt_camera* cam, int delx, int dely, float rot_speed, int zoom) {
+    if (!cam) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "camera_drag: null camera");
+    if (zoom) {
+        // ZOOM_SENSITIVITY 0.25, MIN_FOVY 0.5, MAX_FOVY 100 (Kd-treeConverterMain.h:16-19)
+        cam->fovy_deg += delx * 0.25;
+        if (cam->fovy_deg < 0.5) cam->fovy_deg = 0.5;
+        if (cam->fovy_deg > 100.0) cam->fovy_deg = 100.0;
+        return KDGRT_OK;
+    }
+    const float yaw = delx * rot_speed;
+    float pitch = dely * rot_speed;
+    float m[3][3];
+    gl_rotation(yaw, 0.0f, 1.0f, 0.0f, m);
+    apply(m, cam->u);
+    apply(m, cam->v);
+    apply(m, cam->n);
+    const float ny = cam->n[1];
+    if ((pitch > 0.0f && ny > 0.98f) || (pitch < 0.0f && ny < -0.98f)) pitch = 0.0f;
+    if (pitch != 0.0f) {
+        gl_rotation(pitch, cam->u[0], cam->u[1], cam->u[2], m);
+        apply(m, cam->v);
+        apply(m, cam->n);
+    }
+    reorthonormalise(*cam);
+    return KDGRT_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
 // Proxy meshing (Kd-treeConverterMain.cpp:1860-2099, icosahedron
 // Kd-treeConverterMain.h:128-153): each Gaussian with opacity >= 0.0113 and
 // >= 1/255 becomes 20 triangles whose vertices are the 12 icosahedron

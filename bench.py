@@ -348,6 +348,46 @@ def hybrid_bench(dev, sc, ci, aabb, W, H, stream, steps, warmup):
     return r
 
 
+def viewer_bench(dev, sc, aabb, W, H, frames=300, window=(100, 300)):
+    """The interactive viewer's frame loop (kdgrt_viewer_*) at 1920x1080: the
+    camera orbits the scene centre about the vertical at the reference's
+    0.8 rad/s with 60 frames/s steps (0.76 degrees per frame), each frame is
+    rendered and copied to pinned host memory while the next renders.  FPS as
+    the reference reports it (1000 / render ms, averaged over the window of
+    frames) and the delivered frame rate including the copies."""
+    centre = 0.5 * (aabb[:3].astype(np.float64) + aabb[3:].astype(np.float64))
+    d = np.array([0.48, 0.36, 0.8])
+    d /= np.linalg.norm(d)
+    out = {"workload": f"viewer frame loop: {W}x{H}, orbit 0.8 rad/s at dt 1/60 s about the scene centre, "
+                       f"{frames} frames, FPS window {list(window)}, 3 slots"}
+    for name, fmt in (("rgba8", K.FRAME_RGBA8), ("rgb_f32", K.FRAME_RGB_F32)):
+        cam = K.look_at(centre + 25.0 * d, centre, 60.0, W / H, 0.005)
+        v = K.Viewer(dev, sc, W, H, fmt=fmt, slots=3, window=window)
+        ahead = 0
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            K.camera_orbit(cam, 0.8 / 60.0, tuple(centre), (0.0, 1.0, 0.0))
+            v.submit(cam)
+            ahead += 1
+            if ahead == 2:
+                f, _ = v.acquire(copy=False)
+                v.release(f.frame_id)
+                ahead -= 1
+        while ahead:
+            f, _ = v.acquire(copy=False)
+            v.release(f.frame_id)
+            ahead -= 1
+        wall = time.perf_counter() - t0
+        st = v.stats()
+        v.close()
+        out[name] = {"frame_bytes": W * H * (4 if fmt == K.FRAME_RGBA8 else 12),
+                     "window_avg_fps": round(st["window_avg_fps"], 1),
+                     "mean_render_ms": round(st["mean_render_ms"], 4),
+                     "delivered_fps": round(frames / wall, 1)}
+    log(f"viewer: {out}")
+    return out
+
+
 def orbit_view(aabb, yaw_deg, w=1920, h=1080):
     """view_1080p's camera turned by yaw_deg about the vertical through the
     box centre."""
@@ -463,6 +503,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-multi", action="store_true", help="skip the multi-scene batch (256 views + 4M rays)")
     ap.add_argument("--no-hybrid", action="store_true", help="skip the hybrid (secondary-sphere) view")
+    ap.add_argument("--no-viewer", action="store_true", help="skip the viewer frame loop")
     ap.add_argument("--no-order", action="store_true",
                     help="skip the raster-order and moving-camera legs of the plain view")
     ap.add_argument("--serve-scenes", type=int, default=16)
@@ -590,6 +631,9 @@ def main():
 
     if not args.no_hybrid:
         result["hybrid"] = hybrid_bench(dev, sc, ci, aabb, W, H, stream, args.steps, args.warmup)
+
+    if not args.no_viewer:
+        result["viewer"] = viewer_bench(dev, sc, aabb, W, H)
 
     if not args.no_batched:
         views = views_64(aabb)

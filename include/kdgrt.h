@@ -541,6 +541,100 @@ KDGRT_API kdgrt_status kdgrt_png_write(const char* path, int width, int height, 
  * is kdgrt_render_debug's output (8 int32 per ray, ray order y*width + x). */
 KDGRT_API kdgrt_status kdgrt_stats_write(const int32_t* counters, int width, int height, const char* dir);
 
+/* ---------------- interactive viewer: camera controls ---------------- */
+/* Camera motion of the reference viewer, in its f32 host arithmetic.
+ *
+ * kdgrt_camera_move <- the held-key moves of idle() (Kd-treeConverterMain.cpp:
+ * 3463-3500): with s = speed * dt, w: pos -= n s, s: pos += n s, a: pos -= u s,
+ * d: pos += u s, q: pos += v s, e: pos -= v s, applied in that order for every
+ * key bit set.  *moved (may be NULL) is set when any key was held. */
+#define KDGRT_KEY_W 1
+#define KDGRT_KEY_S 2
+#define KDGRT_KEY_A 4
+#define KDGRT_KEY_D 8
+#define KDGRT_KEY_Q 16
+#define KDGRT_KEY_E 32
+KDGRT_API kdgrt_status kdgrt_camera_move(kdgrt_camera* cam, uint32_t keys, float speed, float dt, int* moved);
+/* kdgrt_camera_orbit <- updateOrbitCamera (Kd-treeConverterMain.cpp:438-480):
+ * pos - centre and the u, v, n axes rotated by `angle` radians about `axis`
+ * (rotateVectorAroundArbitraryAxis, :422-436: Rodrigues' formula with the
+ * axis as given, not normalised), then n normalised, v = n x u, normalised,
+ * u = v x n, normalised (fMyVecNormalize / fMyVecCrossProduct,
+ * MyMathUtility.cpp:59-90).  The reference orbits at 0.8 rad/s about the
+ * scene-specific line centre (0.393, 1.093, 0.698), axis (0.0138, 0.9038,
+ * 0.4278). */
+KDGRT_API kdgrt_status kdgrt_camera_orbit(kdgrt_camera* cam, const float centre[3], const float axis[3],
+                                          float angle);
+/* kdgrt_camera_drag <- mousemove (Kd-treeConverterMain.cpp:1193-1296), a drag
+ * of (delx, dely) pixels (dely positive upwards).  zoom = 0: yaw by
+ * delx * rot_speed degrees about the world y axis (applied to u, v, n), then
+ * pitch by dely * rot_speed degrees about u (applied to v, n), skipped when
+ * the pitch angle is positive and n.y (after the yaw) above 0.98, or
+ * negative and n.y below -0.98; then the axes are re-orthonormalised as in
+ * kdgrt_camera_orbit.  The rotation matrices are
+ * those of glRotatef (the OpenGL specification's axis-angle matrix of the
+ * normalised axis) in f32.  zoom != 0: fovy += delx * 0.25, clamped to
+ * [0.5, 100] degrees.  The reference's rot_speed is 0.1 (CAM_ROT_SPEED). */
+KDGRT_API kdgrt_status kdgrt_camera_drag(kdgrt_camera* cam, int delx, int dely, float rot_speed, int zoom);
+
+/* ---------------- interactive viewer: frame loop ---------------- */
+/* The reference's interactive mode renders each frame on a compute stream
+ * into a display buffer that OpenGL owns (a pixel buffer object registered
+ * with CUDA and mapped on a transfer stream, Kd-treeConverterMain.cpp:
+ * 226-251, 3514-3608) and reports per-frame FPS averaged over frames
+ * [MEASURE_START_FRAME, MEASURE_END_FRAME) = [500, 1000).  An MI355X has no
+ * display, so a viewer here delivers frames to the host: a ring of slots,
+ * each a device framebuffer plus a pinned host frame.  kdgrt_viewer_submit
+ * renders into a free slot on the viewer's compute stream (the scene's tile
+ * schedule applies) and copies the frame to the slot's host memory on a
+ * transfer stream, so the copy of one frame overlaps the render of the next;
+ * kdgrt_viewer_acquire hands out finished frames in submission order, and a
+ * slot is reused only after kdgrt_viewer_release.  Frame formats:
+ *   KDGRT_FRAME_RGB_F32: float3 per pixel, rows as the framebuffer (row
+ *     height-1-y holds image row y), the reference's display-buffer layout;
+ *   KDGRT_FRAME_RGBA8 / KDGRT_FRAME_RGB8: 8-bit pixels, rows top to bottom,
+ *     converted on the device as kdgrt_framebuffer_to_u8 with flip_rows = 1.
+ * A viewer may be driven from two threads (one submitting, one acquiring);
+ * its calls are serialised internally. */
+#define KDGRT_FRAME_RGB_F32 0
+#define KDGRT_FRAME_RGBA8 1
+#define KDGRT_FRAME_RGB8 2
+typedef struct kdgrt_viewer kdgrt_viewer;
+typedef struct kdgrt_viewer_frame {
+    uint64_t frame_id;     /* 0, 1, 2, ... in submission order */
+    const void* pixels;    /* pinned host memory of the slot, valid until release */
+    uint64_t bytes;
+    int width, height, format;
+    float render_ms;       /* GPU time of the frame's render (HIP events) */
+    float fps;             /* 1000 / render_ms, the reference's per-frame FPS */
+    kdgrt_camera camera;   /* the camera the frame was rendered with */
+} kdgrt_viewer_frame;
+typedef struct kdgrt_viewer_stats {
+    uint64_t submitted, delivered;
+    double mean_render_ms;       /* over delivered frames */
+    double window_avg_fps;       /* mean per-frame FPS over delivered frames [window_start, window_end) */
+    uint64_t window_frames;      /* frames of the window delivered so far */
+    double wall_fps;             /* delivered frames / seconds from the first submit to the last delivery */
+} kdgrt_viewer_stats;
+/* n_slots 2..8; stack_depth as in kdgrt_render.  window_start/window_end:
+ * the FPS measurement window in frames (the reference's 500 / 1000). */
+KDGRT_API kdgrt_status kdgrt_viewer_create(kdgrt_context* ctx, const kdgrt_scene* scene, int width, int height,
+                                           int format, int n_slots, int stack_depth, int window_start,
+                                           int window_end, kdgrt_viewer** out);
+/* Renders one frame of `cam` (aspect taken from the camera as given).
+ * KDGRT_ERR_NOT_READY when every slot is still held (not yet acquired and
+ * released) and timeout_ms passed; timeout_ms < 0 waits without limit. */
+KDGRT_API kdgrt_status kdgrt_viewer_submit(kdgrt_viewer* v, const kdgrt_camera* cam, int timeout_ms,
+                                           uint64_t* frame_id);
+/* The oldest submitted frame not yet acquired, once its copy has finished.
+ * KDGRT_ERR_NOT_READY when none finished within timeout_ms (or none is
+ * pending: then at once); timeout_ms < 0 waits without limit. */
+KDGRT_API kdgrt_status kdgrt_viewer_acquire(kdgrt_viewer* v, int timeout_ms, kdgrt_viewer_frame* out);
+KDGRT_API kdgrt_status kdgrt_viewer_release(kdgrt_viewer* v, uint64_t frame_id);
+KDGRT_API kdgrt_status kdgrt_viewer_stats_get(const kdgrt_viewer* v, kdgrt_viewer_stats* out);
+/* Waits for the viewer's device work and frees it (held frames included). */
+KDGRT_API kdgrt_status kdgrt_viewer_destroy(kdgrt_viewer* v);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,7 +23,7 @@ CSRC = os.path.join(ROOT, "3dgrt_kdtree_amd", "csrc")
 INC = os.path.join(ROOT, "include")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HOST = ["host_core.cpp", "sah_builder.cpp", "ingest_stats.cpp"]
-HIP = ["render.hip", "autotune.hip", "server.hip", "host_api.hip"]
+HIP = ["render.hip", "autotune.hip", "server.hip", "host_api.hip", "viewer.hip"]
 
 
 def _run_all(cmds, cwd):
