@@ -739,16 +739,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 4 
 // it runs.
 // ---------------------------------------------------------------------------
 // The host appends request records to a table in pinned host memory and
-// publishes them with a release store of the request count.  Waves claim
-// 8x8-pixel tiles from one counter in device memory, in request order.  A
-// claimed tile may not be published yet: the wave that holds the first
-// unpublished tile polls the host table, copies new records into device
-// memory and then advances the device's published-tile mark; every other
-// wave waits on that mark in device memory.  The wave that finishes a
+// publishes them with a release store of the request count.  The tile space
+// of the epoch (requests in order, each a whole number of 64-tile groups of
+// 8x8-pixel tiles) is dealt to kXcds queues as the batch kernel deals it:
+// group g belongs to queue g mod kXcds, so the neighbouring tiles of a group
+// run on one XCD and share its L2.  A wave claims the next tile of its own
+// XCD's queue with one atomic on that queue's counter.  A claimed tile may
+// not be published yet: the wave that holds the first tile of its queue's
+// first unpublished group (at most one per queue) polls the host table,
+// copies new records into device memory and then raises the device's
+// published-tile mark (atomic maximum, so concurrent pollers never lower
+// it); every other waiting wave takes published tiles that other queues have
+// not handed out (compare-and-swap on their counters, never an unpublished
+// one), or waits on the mark in device memory.  The wave that finishes a
 // request's last tile stores the request's sequence number into the host's
 // done table.  The host stops an epoch by storing 1 to `stop` after its last
-// publication; the polling wave turns it into a device stop flag and every
-// wave exits when it holds no published tile.  Every wait is bounded
+// publication; a polling wave turns it into a device stop flag and every
+// wave exits once its claim is unpublished and no other queue has a
+// published tile left for it.  Every wait is bounded
 // (spin_limit polls): a wave that runs out of polls exits too, and the host
 // publishes whatever an early-ended epoch left unfinished again.  Records are
 // written once per epoch (no reuse), so a record below the published mark
@@ -783,13 +791,22 @@ struct ServeHostCtl { // pinned host memory, written by the host
     uint32_t pad[14];
 };
 
+// Device control words.  Every wave claims a tile through its queue's
+// counter, and agent-scope accesses from all XCDs to one line serialise at
+// the memory side, so each counter has a 128-B line of its own and the
+// marks another.
+struct ServeQueueWord {
+    uint32_t next; // claims of the queue (queue_tile maps claim j of queue q to its tile)
+    uint32_t pad[31];
+};
 struct ServeDevCtl { // device memory
-    uint32_t next_tile;
-    uint32_t mirror_tiles; // tiles of the records copied to the device
+    uint32_t mirror_tiles; // tiles of the records copied to the device (a multiple of 64)
     uint32_t mirror_req;   // records copied to the device
     uint32_t stop;
-    uint32_t pad[12];
+    uint32_t pad[29];
+    ServeQueueWord queue[kXcds];
 };
+static_assert(sizeof(ServeDevCtl) == 128 * (kXcds + 1), "ServeDevCtl layout");
 
 struct ServeArgs {
     ServeHostCtl* hctl;
@@ -849,96 +866,179 @@ __device__ void serve_poll_host(const ServeArgs& a, int lane) {
             end = ld_system(&last->tile_begin) + ld_system(&last->tile_count);
         }
         wave_accesses_done();
+        // pollers may overlap: each copied every record below its marks, and
+        // the maxima keep the marks from going back
         if (lane == 0) {
-            st_agent(&a.dctl->mirror_req, pr);
+            __hip_atomic_fetch_max(&a.dctl->mirror_req, pr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             wave_accesses_done();
-            st_agent(&a.dctl->mirror_tiles, end);
+            __hip_atomic_fetch_max(&a.dctl->mirror_tiles, end, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     } else if (st) {
         if (lane == 0) st_agent(&a.dctl->stop, 1u);
     }
 }
 
+// The request whose tiles hold tile t (records below the device mark are
+// complete and sorted by tile_begin), starting from the request k of the
+// wave's previous tile: forward while t lies past it, by bisection when a
+// stolen tile lies before it.  max_req when t lies in no record.
+__device__ __forceinline__ uint32_t serve_find(const ServeArgs& a, uint32_t t, uint32_t k) {
+    auto begin = [&](uint32_t i) { return wave_u32(ld_agent(reinterpret_cast<const uint32_t*>(a.drec + i))); };
+    if (k < (uint32_t)a.max_req && t < begin(k)) {
+        uint32_t lo = 0, hi = k; // the answer is below k
+        while (lo + 1 < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (begin(mid) <= t) lo = mid;
+            else hi = mid;
+        }
+        k = lo;
+    }
+    while (k < (uint32_t)a.max_req) {
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(a.drec + k);
+        const uint32_t b = wave_u32(ld_agent(w)), c = wave_u32(ld_agent(w + 1));
+        if (t < b + c) break;
+        ++k;
+    }
+    return k;
+}
+
+// One tile of a request: trace its rays, write the pixels through to memory,
+// count the tile.  r is the record of the request k that holds tile t.
+template <int K>
+__device__ __forceinline__ void serve_tile(const ServeArgs& a, const ServeRec& r, uint32_t k, uint32_t t, int lane,
+                                           uint2* lds_stack) {
+    const uint32_t local = t - r.tile_begin;
+    const uint32_t grp = local / 64u, gl = local % 64u;
+    const int tx = (int)((grp % (uint32_t)r.groups_x) * 8u + gl % 8u);
+    const int ty = (int)((grp / (uint32_t)r.groups_x) * 8u + gl / 8u);
+    if (tx < r.tiles_x && ty < r.tiles_y && r.scene < (uint32_t)a.n_scenes) {
+        const DevScene sc = a.scenes[r.scene];
+        const int x = tx * 8 + (lane & 7), y = ty * 8 + (lane >> 3);
+        if (x < r.W && y < r.H) {
+            float o[3], d[3], rgb[3], A;
+            make_ray(r.cam, x, y, o, d);
+            trace_ray<K, false>(sc, o, d, a.depth, lds_stack, lane, rgb, A, nullptr);
+            uint32_t* px = reinterpret_cast<uint32_t*>(a.arena + r.out_off) + 4 * ((size_t)(r.H - y - 1) * r.W + x);
+            st_system(px + 0, __float_as_uint(rgb[0]));
+            st_system(px + 1, __float_as_uint(rgb[1]));
+            st_system(px + 2, __float_as_uint(rgb[2]));
+            st_system(px + 3, __float_as_uint(1.0f - A));
+        }
+    }
+    // the tile's pixels are written through before it counts as finished
+    wave_accesses_done();
+    if (lane == 0) {
+        const uint32_t c = __hip_atomic_fetch_add(&a.done_tiles[k], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (c + 1 == r.tile_count) __hip_atomic_store(&a.hdone[k], r.seq + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+// The record of the request that holds published tile t, kept across tiles:
+// most tiles of a wave belong to the request of its previous tile.
+__device__ __forceinline__ bool serve_record(const ServeArgs& a, uint32_t t, uint32_t& k, ServeRec& r, bool& have,
+                                             int lane) {
+    if (have && t >= r.tile_begin && t - r.tile_begin < r.tile_count) return true;
+    k = serve_find(a, t, have ? k : 0u);
+    if (k >= (uint32_t)a.max_req) return false; // malformed table: never read past it
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(a.drec + k);
+    const uint32_t mine = lane < (int)kRecWords ? ld_agent(src + lane) : 0u;
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&r);
+#pragma unroll
+    for (int w = 0; w < (int)kRecWords; ++w) dst[w] = __builtin_amdgcn_readlane(mine, w);
+    have = true;
+    return true;
+}
+
+// A published, unclaimed tile of another queue, taken only if it is
+// published (compare-and-swap on the queue's counter), or ~0u.
+__device__ __forceinline__ uint32_t serve_steal(const ServeArgs& a, uint32_t home, uint32_t mt, int lane) {
+    uint32_t got = ~0u;
+    if (lane == 0) {
+        for (uint32_t j = 1; j < (uint32_t)kXcds && got == ~0u; ++j) {
+            const uint32_t q = (home + j) % kXcds;
+            uint32_t n = ld_agent(&a.dctl->queue[q].next);
+            while (queue_tile(q, n) < mt) {
+                if (__hip_atomic_compare_exchange_strong(&a.dctl->queue[q].next, &n, n + 1u, __ATOMIC_RELAXED,
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    got = queue_tile(q, n);
+                    break;
+                }
+            }
+        }
+    }
+    return wave_u32(got);
+}
+
 template <int K>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_serve(ServeArgs a) {
     const int lane = threadIdx.x;
     extern __shared__ uint2 lds_stack[];
-    uint32_t k = 0; // the request of the last tile (tiles are claimed in increasing order)
+    const uint32_t home = blockIdx.x % kXcds;
+    uint32_t k = 0, mt_seen = 0; // the request of the wave's last tile; the last published mark read
+    ServeRec r;
+    bool have = false;
+    uint32_t own = 0;    // the wave's claim on its home queue
+    bool holding = false;
+    uint32_t spins = 0;
     while (true) {
-        uint32_t t = 0;
-        if (lane == 0) t = __hip_atomic_fetch_add(&a.dctl->next_tile, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        t = wave_u32(t);
-        bool quit = false;
-        for (uint32_t spins = 0;; ++spins) {
-            uint32_t mt = 0, stop = 0;
+        if (!holding) {
+            // claim the home queue's next tile (one atomic on the queue's line)
+            uint32_t c = 0;
+            if (lane == 0)
+                c = __hip_atomic_fetch_add(&a.dctl->queue[home].next, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            own = queue_tile(home, wave_u32(c));
+            holding = true;
+            spins = 0;
+        }
+        uint32_t t = ~0u; // the tile to run now
+        // marks only grow: a claim below the last mark read is published
+        if (own < mt_seen) {
+            t = own;
+            holding = false;
+        } else {
+            uint32_t stop = 0;
             if (lane == 0) {
-                mt = ld_agent(&a.dctl->mirror_tiles);
+                mt_seen = ld_agent(&a.dctl->mirror_tiles);
                 stop = ld_agent(&a.dctl->stop);
             }
-            mt = wave_u32(mt);
+            mt_seen = wave_u32(mt_seen);
             stop = wave_u32(stop);
-            if (t < mt) break;
-            if (stop || spins >= a.spin_limit) {
-                quit = true;
+            if (own < mt_seen) {
+                t = own;
+                holding = false;
+            } else if (stop || spins >= a.spin_limit) {
                 break;
-            }
-            if (t == mt) {
+            } else if ((own % kQueueGroup) == 0 && own / kQueueGroup < mt_seen / kQueueGroup + kXcds) {
+                // the first tile of its queue's first unpublished group (at
+                // most one per queue, within kXcds groups of the mark)
+                // polls the host
                 serve_poll_host(a, lane);
                 __builtin_amdgcn_s_sleep(2);
+                ++spins;
+                continue;
             } else {
-                // waiting behind the polling wave: back off up to ~30 us
-                const uint32_t naps = spins < 4 ? 1u : (spins < 64 ? 4u : 16u);
-                for (uint32_t z = 0; z < naps; ++z) __builtin_amdgcn_s_sleep(127);
+                // while its own claim waits, a wave runs published tiles that
+                // other queues have not handed out yet (the end of a burst)
+                t = serve_steal(a, home, mt_seen, lane);
+                if (t == ~0u) {
+                    // nothing to take: back off up to ~30 us
+                    const uint32_t naps = spins < 4 ? 1u : (spins < 64 ? 4u : 16u);
+                    for (uint32_t z = 0; z < naps; ++z) __builtin_amdgcn_s_sleep(127);
+                    ++spins;
+                    continue;
+                }
+                spins = 0;
             }
         }
-        if (quit) break;
-        // the request of tile t: records below the device mark are complete
-        while (k < (uint32_t)a.max_req) {
-            const uint32_t* w = reinterpret_cast<const uint32_t*>(a.drec + k);
-            const uint32_t b = wave_u32(ld_agent(w)), c = wave_u32(ld_agent(w + 1));
-            if (t < b + c) break;
-            ++k;
-        }
-        if (k >= (uint32_t)a.max_req) break; // malformed table: never read past it
-        ServeRec r;
-        {
-            const uint32_t* src = reinterpret_cast<const uint32_t*>(a.drec + k);
-            const uint32_t mine = lane < (int)kRecWords ? ld_agent(src + lane) : 0u;
-            uint32_t* dst = reinterpret_cast<uint32_t*>(&r);
-#pragma unroll
-            for (int w = 0; w < (int)kRecWords; ++w) dst[w] = __builtin_amdgcn_readlane(mine, w);
-        }
-        const uint32_t local = t - r.tile_begin;
-        const uint32_t grp = local / 64u, gl = local % 64u;
-        const int tx = (int)((grp % (uint32_t)r.groups_x) * 8u + gl % 8u);
-        const int ty = (int)((grp / (uint32_t)r.groups_x) * 8u + gl / 8u);
-        if (tx < r.tiles_x && ty < r.tiles_y && r.scene < (uint32_t)a.n_scenes) {
-            const DevScene sc = a.scenes[r.scene];
-            const int x = tx * 8 + (lane & 7), y = ty * 8 + (lane >> 3);
-            if (x < r.W && y < r.H) {
-                float o[3], d[3], rgb[3], A;
-                make_ray(r.cam, x, y, o, d);
-                trace_ray<K, false>(sc, o, d, a.depth, lds_stack, lane, rgb, A, nullptr);
-                uint32_t* px = reinterpret_cast<uint32_t*>(a.arena + r.out_off) + 4 * ((size_t)(r.H - y - 1) * r.W + x);
-                st_system(px + 0, __float_as_uint(rgb[0]));
-                st_system(px + 1, __float_as_uint(rgb[1]));
-                st_system(px + 2, __float_as_uint(rgb[2]));
-                st_system(px + 3, __float_as_uint(1.0f - A));
-            }
-        }
-        // the tile's pixels are written through before it counts as finished
-        wave_accesses_done();
-        if (lane == 0) {
-            const uint32_t c = __hip_atomic_fetch_add(&a.done_tiles[k], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (c + 1 == r.tile_count) __hip_atomic_store(&a.hdone[k], r.seq + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-        }
+        if (!serve_record(a, t, k, r, have, lane)) break;
+        serve_tile<K>(a, r, k, t, lane, lds_stack);
     }
 }
 
 __global__ void k_serve_reset(ServeDevCtl* c, uint32_t* done_tiles, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
-        c->next_tile = 0;
+        for (int q = 0; q < kXcds; ++q) c->queue[q].next = 0;
         c->mirror_tiles = 0;
         c->mirror_req = 0;
         c->stop = 0;
