@@ -755,8 +755,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 4 
 // request's last tile stores the request's sequence number into the host's
 // done table.  The host stops an epoch by storing 1 to `stop` after its last
 // publication; a polling wave turns it into a device stop flag and every
-// wave exits once its claim is unpublished and no other queue has a
-// published tile left for it.  Every wait is bounded
+// wave exits once its own claim is unpublished (claims of a queue are
+// consecutive, so its published tiles have all been handed out by then and
+// the waves holding them finish them).  Every wait is bounded
 // (spin_limit polls): a wave that runs out of polls exits too, and the host
 // publishes whatever an early-ended epoch left unfinished again.  Records are
 // written once per epoch (no reuse), so a record below the published mark
