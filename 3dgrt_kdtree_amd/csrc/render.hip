@@ -503,7 +503,7 @@ struct BatchArgs {
     const WorkItem* items;
     int n_items;
     uint32_t q_len[kXcds];       // tiles owned by each queue (queue_tile maps them)
-    uint32_t* counters;          // kXcds counters
+    uint32_t* counters;          // kXcds counters, kCounterStride words apart
     int depth;
     int tile_w, tile_h;          // view tile in pixels; tile_w*tile_h is a multiple of 64
 };
@@ -646,8 +646,14 @@ __global__ __launch_bounds__(SORT ? kCinBlock : kCinBlockSmall) void k_cone_prep
     }
 }
 
+// Batch queue counters, one per 128-B line: every wave of the chip claims
+// its tiles through them, and atomics from all XCDs on one line serialise at
+// the memory side.
+constexpr int kCounterStride = 32;
+constexpr size_t kCounterBytes = sizeof(uint32_t) * kCounterStride * kXcds;
+
 __global__ void k_reset(uint32_t* c, int n) {
-    if ((int)threadIdx.x < n) c[threadIdx.x] = 0;
+    if ((int)threadIdx.x < n) c[threadIdx.x * kCounterStride] = 0;
 }
 
 template <int K>
@@ -661,7 +667,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 4 
         const int q = (home + probe) % kXcds;
         const uint32_t qlen = a.q_len[q];
         uint32_t t = 0;
-        if (lane == 0) t = atomicAdd(&a.counters[q], 1u);
+        if (lane == 0) t = atomicAdd(&a.counters[q * kCounterStride], 1u);
         // readfirstlane makes the tile (and everything derived from it)
         // provably wave-uniform, so the work item lives in scalar registers
         t = __builtin_amdgcn_readfirstlane(t);
@@ -2416,9 +2422,9 @@ kdgrt_status batch_build(kdgrt_context* ctx, const kdgrt_view* views, int n_view
         }
     }
     if (e == hipSuccess) e = hipMalloc(&b->d_items, items.size() * sizeof(WorkItem));
-    if (e == hipSuccess) e = hipMalloc(&b->d_counters, 64 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&b->d_counters, kCounterBytes);
     if (e == hipSuccess) e = hipMemcpy(b->d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(b->d_counters, 0, 64 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(b->d_counters, 0, kCounterBytes);
     if (e != hipSuccess) {
         hipFree(b->d_items);
         hipFree(b->d_counters);
@@ -2470,7 +2476,7 @@ kdgrt_status slot_create(kdgrt_context* ctx, int max_items, BatchSlot** out) {
     sl->max_items = max_items;
     hipError_t e = hipHostMalloc((void**)&sl->h_items, sizeof(WorkItem) * max_items, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc(&sl->d_items, sizeof(WorkItem) * max_items);
-    if (e == hipSuccess) e = hipMalloc(&sl->d_counters, 64 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&sl->d_counters, kCounterBytes);
     if (e != hipSuccess) {
         slot_destroy(sl);
         return set_error(KDGRT_ERR_OUT_OF_MEMORY, std::string("slot_create: ") + hipGetErrorString(e));
