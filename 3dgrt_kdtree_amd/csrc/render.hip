@@ -84,7 +84,7 @@ static inline uint32_t interleave_grid(uint64_t n_logical) {
 }
 
 template <int K, bool DBG, int FMT, bool PROF = false, bool CONE = true>
-__global__ __launch_bounds__(64) void k_view(ViewArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DBG || PROF ? 1 : 4))) void k_view(ViewArgs a) {
     // One wavefront per block = one 8x8-pixel tile, so a finished wave frees
     // its slot at once (per-ray cost varies widely across an image).  Groups
     // of 64 tiles (64x64 px) share an XCD, unless the launch follows a tile
@@ -924,7 +924,9 @@ __device__ __forceinline__ void serve_tile(const ServeArgs& a, const ServeRec& r
         if (x < r.W && y < r.H) {
             float o[3], d[3], rgb[3], A;
             make_ray(r.cam, x, y, o, d);
-            trace_ray<K, false>(sc, o, d, a.depth, lds_stack, lane, rgb, A, nullptr);
+            // without the look-ahead: with it this kernel's extra state spills
+            // (36 B per lane) and the 64-view burst takes 50.3 instead of 49.0 ms
+            trace_ray<K, false, false, false, false, false>(sc, o, d, a.depth, lds_stack, lane, rgb, A, nullptr);
             uint32_t* px = reinterpret_cast<uint32_t*>(a.arena + r.out_off) + 4 * ((size_t)(r.H - y - 1) * r.W + x);
             st_system(px + 0, __float_as_uint(rgb[0]));
             st_system(px + 1, __float_as_uint(rgb[1]));

@@ -470,6 +470,38 @@ __device__ __forceinline__ bool next_leaf(RayState& st, const DevScene& sc, int 
     return (++st.guard <= kLeafGuard) && (st.A < 0.99f);
 }
 
+// The step after a leaf without the opacity test (cudaRenderer.cu:1690-1708):
+// false when the leaf's segment reaches the scene's far end (no leaf follows)
+// or the leaf guard runs out; otherwise pop or restart at the root.  The
+// opacity test, which needs the leaf composited, is made after it.
+__device__ __forceinline__ bool advance(RayState& st, const DevScene& sc, int depth, const uint32_t* __restrict__ stk) {
+    if (st.t_far >= st.t_scene_far) return false;
+    if (st.quant == 0) {
+        st.node = sc.nodes[0];
+        st.t_near = st.t_far;
+        st.t_far = st.t_scene_far;
+    } else {
+        st.node = make_uint2(stk[64 * st.top], stk[64 * (depth + st.top)]);
+        const float tf = __uint_as_float(stk[64 * (2 * depth + st.top)]);
+        if (st.top == 0) st.top = (uint32_t)depth;
+        --st.top;
+        --st.quant;
+        st.t_near = st.t_far;
+        st.t_far = tf;
+    }
+    return ++st.guard <= kLeafGuard;
+}
+
+// One iteration of the descent (one quad record, up to two steps).
+__device__ __forceinline__ void descend_once(RayState& st, const DevScene& sc, int depth, uint32_t* __restrict__ stk) {
+    const uint4* qp = sc.quads + 3 * (size_t)(st.node.x >> 3);
+    const uint4 q0 = qp[0], q1 = qp[1], q2 = qp[2];
+    const uint32_t pick = node_step<false>(st, make_uint2(q0.x, q0.y), make_uint2(q0.z, q0.w), depth, stk, nullptr);
+    if ((st.node.x & 7u) == 3u) return;
+    const uint4 g = pick ? q2 : q1;
+    node_step<false>(st, make_uint2(g.x, g.y), make_uint2(g.z, g.w), depth, stk, nullptr);
+}
+
 // The leaf in st.node (cudaRenderer.cu:1576-1688): the leaf's candidates in
 // stored order, exact test, stable order by t, composite with the skip rule
 // and early termination.  The K-entry register buffer holds the leaf's hits
@@ -673,10 +705,12 @@ __device__ __forceinline__ void leaf(RayState& st, const DevScene& sc, RayDbg* d
 
 // Per-wavefront LDS of the trace: the lanes' short-stack rings, depth
 // entries of 12 B (node words x, y and t_far, each entry-major over the 64
-// lanes): 768 * depth bytes per wavefront; the DBG variant adds a fourth
-// plane (the pushed nodes' indices), 1024 * depth bytes.
+// lanes): 768 * depth bytes per wavefront, plus a 16-B stash per lane
+// (1024 B) where a lane parks the traversal state of its next leaf while it
+// composites the current one; the DBG variant, which does not look ahead,
+// has a fourth plane instead (the pushed nodes' indices), 1024 * depth bytes.
 __host__ __device__ constexpr size_t wave_lds_bytes(int depth, bool dbg = false) {
-    return (size_t)depth * (dbg ? 1024u : 768u);
+    return dbg ? (size_t)depth * 1024u : (size_t)depth * 768u + 1024u;
 }
 
 // Whole ray: outer loop of cudaRenderer.cu:1552-1709.  CONE: the ray starts
@@ -688,7 +722,7 @@ __host__ __device__ constexpr size_t wave_lds_bytes(int depth, bool dbg = false)
 // not reset them between segments), t_scene_far is clamped to max_t, and
 // cone_rt selects the cone pre-test at run time (the first segment of a view
 // ray) while later segments use the sphere pre-test.
-template <int K, bool DBG, bool PROF = false, bool CONE = false, bool HYB = false>
+template <int K, bool DBG, bool PROF = false, bool CONE = false, bool HYB = false, bool AHEAD = true>
 __device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], const float d[3], int depth,
                                           uint2* __restrict__ wave_lds, int lane, float rgb[3], float& A,
                                           RayDbg* dbg, float max_t = 0.0f, bool cone_rt = false) {
@@ -709,7 +743,20 @@ __device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], 
             go = !(st.t_near > tf) && (st.A < 0.99f);
         }
     }
-    if (go) {
+    // Look-ahead (AHEAD; never in the counting, profiling and hybrid variants): the
+    // lanes of a wave descend in lock step and wait for the lane with the
+    // longest descent before any of them composites its leaf.  A lane that
+    // has found its leaf takes it (its node and t range), makes the step
+    // after it at once -- the pop or root restart depends only on the stack
+    // and the leaf's t_far, not on the composite -- and goes on descending
+    // towards its next leaf while other lanes still descend, so the next
+    // descent phase starts part way or not at all.  The opacity test follows
+    // the composite as before; a ray that terminates has only spent
+    // iterations in which its lane would have idled.  Each lane visits the
+    // same leaves in the same order, so results are unchanged.  While it
+    // composites, a lane parks the state of its next descent in its LDS stash.
+    constexpr bool kAhead = AHEAD && !DBG && !PROF && !HYB;
+    if (go && !kAhead) {
         while (true) {
             descend<DBG>(st, sc, depth, stk, dbg);
             KD_PROF_MARK(0);
@@ -717,6 +764,45 @@ __device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], 
             const bool more = next_leaf<DBG>(st, sc, depth, stk, dbg);
             KD_PROF_MARK(4);
             if (!more) break;
+        }
+    } else if (kAhead) {
+        // the stash holds whichever state is not being worked on: the taken
+        // leaf while the lane descends, the next descent while it composites
+        uint32_t* const stash = stk + 64 * (3 * depth);
+        bool alive = go, have = false, last = false;
+        while (__any(alive)) {
+            while (true) {
+                if (alive && !have && (st.node.x & 7u) == 3u) {
+                    stash[0] = st.node.x;
+                    stash[64] = st.node.y;
+                    stash[128] = __float_as_uint(st.t_near);
+                    stash[192] = __float_as_uint(st.t_far);
+                    have = true;
+                    last = !advance(st, sc, depth, stk);
+                }
+                if (!__any(alive && !have)) break;
+                if (alive && !(have && last) && (st.node.x & 7u) != 3u) descend_once(st, sc, depth, stk);
+            }
+            if (alive) {
+                const uint32_t lx = stash[0], ly = stash[64], ltn = stash[128], ltf = stash[192];
+                stash[0] = st.node.x;
+                stash[64] = st.node.y;
+                stash[128] = __float_as_uint(st.t_near);
+                stash[192] = __float_as_uint(st.t_far);
+                st.node = make_uint2(lx, ly);
+                st.t_near = __uint_as_float(ltn);
+                st.t_far = __uint_as_float(ltf);
+                // the parked values are read back from LDS, not carried in
+                // registers through the composite
+                asm volatile("" ::: "memory");
+                leaf<K, DBG, PROF, CONE, HYB>(st, sc, dbg);
+                asm volatile("" ::: "memory");
+                st.node = make_uint2(stash[0], stash[64]);
+                st.t_near = __uint_as_float(stash[128]);
+                st.t_far = __uint_as_float(stash[192]);
+                have = false;
+                if (!(st.A < 0.99f) || last) alive = false;
+            }
         }
     }
     rgb[0] = st.rgb[0];
