@@ -174,7 +174,8 @@ KDGRT_API kdgrt_status kdgrt_timer_end(kdgrt_context* ctx, void* stream, float* 
 KDGRT_API kdgrt_status kdgrt_device_info(kdgrt_context* ctx, char* name, size_t name_len,
                                          int* n_cu, size_t* total_mem, size_t* lds_per_block);
 
-/* Streaming device copy of n_float4 x 16 B (grid-stride, 256-thread blocks).
+/* Streaming device copy of n_float4 x 16 B (256-thread blocks, four 16-B
+ * nontemporal loads then four stores per thread and chunk).
  * No reference counterpart: the calibration kernel whose HBM counter
  * readings (FETCH_SIZE, WRITE_SIZE) scale the render kernel's. */
 KDGRT_API kdgrt_status kdgrt_copy_f32x4(kdgrt_context* ctx, void* dst, const void* src, uint64_t n_float4,

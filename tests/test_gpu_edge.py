@@ -116,3 +116,21 @@ def test_rays_missing_the_box(dev):
     _same(gpu, ref, "missing rays")
     assert (gpu["rgbt"] == np.float32([0, 0, 0, 1])).all() and (gpu["counters"] == 0).all()
     sc.free()
+
+
+@pytest.mark.parametrize("n", [0, 1, 37, 1024, 1024 * 5 + 37, 256 * 1024 + 1023])
+def test_copy_kernel_exact(dev, n):
+    """k_copy (the HBM-ceiling and counter-calibration kernel) copies every
+    16-B word: whole chunks of 4 x 256 words and the word-by-word remainder."""
+    import ctypes as C
+    rng = np.random.default_rng(n)
+    src_h = rng.integers(0, 2**32, size=(max(n, 1), 4), dtype=np.uint32)
+    src = dev.upload(src_h)
+    dst = dev.upload(np.zeros((max(n, 1) + 1, 4), np.uint32))  # one guard word past the end
+    K.check(K.lib().kdgrt_copy_f32x4(dev.h, dst.ptr, src.ptr, C.c_uint64(n), None))
+    dev.sync()
+    got = dst.download(np.uint32, (max(n, 1) + 1, 4))
+    assert np.array_equal(got[:n], src_h[:n]), "copied words differ"
+    assert not got[n:].any(), "words past the copy were written"
+    src.free()
+    dst.free()
