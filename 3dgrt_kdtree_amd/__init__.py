@@ -111,7 +111,7 @@ class ServerConfig(C.Structure):
                 ("max_inflight_batches", C.c_int32), ("plain_launches", C.c_int32), ("tuning", Tuning),
                 ("batch_window_ms", C.c_double), ("simulate_mrays_per_s", C.c_double),
                 ("persistent_kernel", C.c_int32), ("persistent_max_requests", C.c_int32),
-                ("epoch_idle_ms", C.c_double), ("persistent_spin_limit", C.c_uint32)]
+                ("epoch_idle_ms", C.c_double), ("persistent_spin_limit", C.c_uint32), ("idle_plain", C.c_int32)]
 
 
 class Request(C.Structure):
@@ -784,7 +784,7 @@ class Server:
 
     def __init__(self, dev, budget_bytes, max_batch_views=16, max_inflight=2, plain=False, tuning=None,
                  window_ms=1.0, simulate_mrays_per_s=0.0, persistent=False, epoch_max_requests=0, epoch_idle_ms=5.0,
-                 spin_limit=0):
+                 spin_limit=0, idle_plain=False):
         cfg = ServerConfig()
         cfg.hbm_budget_bytes = int(budget_bytes)
         cfg.max_batch_views = int(max_batch_views)
@@ -797,6 +797,7 @@ class Server:
         cfg.persistent_max_requests = int(epoch_max_requests)
         cfg.epoch_idle_ms = float(epoch_idle_ms)
         cfg.persistent_spin_limit = int(spin_limit)
+        cfg.idle_plain = 1 if idle_plain else 0
         self.h = C.c_void_p()
         check(lib().kdgrt_server_create(dev.h if dev is not None else None, C.byref(cfg), C.byref(self.h)))
 

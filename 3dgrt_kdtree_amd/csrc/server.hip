@@ -26,6 +26,9 @@
 //    the persistent kernel).  An idle epoch is stopped after epoch_idle_ms
 //    and the next request starts a new one; requests an epoch leaves
 //    unfinished (its kernel ended early) are published again in the next.
+//    With idle_plain = 1, a request that finds the device idle (no epoch, no
+//    other request) runs as one plain one-view launch instead, and no epoch
+//    starts until that launch has finished.
 //  * plain_launches = 1 is the comparison mode: each request is one plain
 //    one-view launch (k_view), run one at a time, as the reference renders.
 //  * With ctx == NULL and simulate_mrays_per_s > 0 the same scheduler runs
@@ -245,10 +248,12 @@ bool launch_one(kdgrt_server* s) {
     std::vector<Req*> take;
     while (!s->pending.empty() && (int)take.size() < maxv) {
         Req* r = s->pending.front();
-        uint64_t off = 0;
-        if (!s->arena.alloc(r->bytes, &off)) break; // arrival order: no overtaking
-        r->off = off;
-        r->has_mem = true;
+        if (!r->has_mem) {
+            uint64_t off = 0;
+            if (!s->arena.alloc(r->bytes, &off)) break; // arrival order: no overtaking
+            r->off = off;
+            r->has_mem = true;
+        }
         take.push_back(r);
         s->pending.pop_front();
     }
@@ -318,6 +323,7 @@ bool launch_one(kdgrt_server* s) {
 bool persist_step(kdgrt_server* s) {
     bool changed = false;
     const double t = s->now();
+    retire(s, &changed); // plain launches of requests that found the device idle
     for (size_t i = 0; i < s->serving.size();) {
         Req* r = s->serving[i].first;
         if (!kdgrt::serve_done(s->sq, s->serving[i].second)) {
@@ -368,7 +374,16 @@ bool persist_step(kdgrt_server* s) {
             ++it;
         }
     }
-    while (!s->pending.empty() && !s->epoch_stopping) {
+    // idle_plain: a request that finds the device idle -- no epoch running,
+    // no plain launch in flight, no other request waiting -- runs as one
+    // plain one-view launch (cone pre-test, tile order) on a slot's stream.
+    // No epoch starts while such a launch runs: the persistent kernel's
+    // waves would hold the wave slots that the launch's remaining
+    // workgroups wait for.  Requests that arrive meanwhile wait and then
+    // start an epoch together (or run plain again if alone).
+    if (s->cfg.idle_plain && !s->epoch_running && s->flights.empty() && s->pending.size() == 1 && launch_one(s))
+        changed = true;
+    while (!s->pending.empty() && !s->epoch_stopping && s->flights.empty()) {
         Req* r = s->pending.front();
         if (!r->has_mem) {
             uint64_t off = 0;
@@ -422,9 +437,13 @@ void worker_main_persistent(kdgrt_server* s) {
     std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
         if (persist_step(s)) s->cv_done.notify_all();
-        if (s->stopping && s->pending.empty() && s->serving.empty() && !s->epoch_running) break;
-        // completions are flags in host memory: poll them briskly while work runs
-        const double wait_s = (!s->serving.empty() || s->epoch_stopping) ? 10e-6 : (s->epoch_running ? 200e-6 : -1.0);
+        if (s->stopping && s->pending.empty() && s->serving.empty() && !s->epoch_running && s->flights.empty())
+            break;
+        // completions are flags in host memory (epochs) or events (plain
+        // launches): poll them briskly while work runs
+        const double wait_s = (!s->serving.empty() || s->epoch_stopping || !s->flights.empty())
+                                  ? 10e-6
+                                  : (s->epoch_running ? 200e-6 : -1.0);
         if (wait_s > 0) s->cv_work.wait_for(lk, std::chrono::duration<double>(wait_s));
         else s->cv_work.wait(lk);
     }

@@ -288,7 +288,8 @@ def multi_scene_batch(dev, scenes, aabbs, counts, tuning, n_views=256, n_rays=4_
 def serve_bench(dev, scenes, aabbs, counts, tuning, rates=((200, 4.0), (1000, 2.0)), window_ms=0.0):
     """Serving: the resident scenes (HBM budget = resident bytes / 0.8),
     Poisson arrivals, one view per request, in three modes: continuous
-    batching into one running persistent kernel ("persistent"), overlapping
+    batching into one running persistent kernel ("persistent"; a request
+    that finds the device idle runs as one plain launch), overlapping
     batched launches on three streams ("batched"), and plain one-launch-per-
     request one at a time ("plain", the reference's way)."""
     resident = sum(sc.device_bytes() for sc in scenes)
@@ -299,7 +300,7 @@ def serve_bench(dev, scenes, aabbs, counts, tuning, rates=((200, 4.0), (1000, 2.
            "hbm_budget_bytes": budget, "batch_window_ms": window_ms, "max_batch_views": 32, "inflight_batches": 3}
     for mode in ("persistent", "batched", "plain"):
         srv = K.Server(dev, budget, max_batch_views=32, max_inflight=3, plain=(mode == "plain"), tuning=tuning,
-                       window_ms=window_ms, persistent=(mode == "persistent"))
+                       window_ms=window_ms, persistent=(mode == "persistent"), idle_plain=(mode == "persistent"))
         ids = [srv.add_resident_scene(sc) for sc in scenes]
         srv.start()
         for lam, dur in rates:

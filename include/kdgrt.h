@@ -427,6 +427,10 @@ typedef struct kdgrt_server_config {
     int32_t persistent_max_requests; /* requests per epoch (0: 4096); a full epoch drains and a new one starts */
     double epoch_idle_ms;            /* an idle epoch's kernel is stopped after this long */
     uint32_t persistent_spin_limit;  /* polls a kernel wave makes before it gives up waiting (0: 4M) */
+    /* persistent mode: 1 = a request that finds the device idle (no epoch
+     * running, no plain launch in flight, no other request pending) runs as
+     * one plain one-view launch; no epoch starts until it has finished */
+    int32_t idle_plain;
 } kdgrt_server_config;
 
 typedef struct kdgrt_request {

@@ -142,3 +142,27 @@ def test_persistent_waves_that_give_up_early(dev, three_scenes):
     results, st = _serve(dev, scs, reqs, gaps=gaps, spin_limit=200)
     _check(dev, scs, reqs, results, "early")
     assert st["completed"] == len(reqs)
+
+
+def test_persistent_idle_plain(dev, three_scenes):
+    """idle_plain: a request that finds the device idle runs as one plain
+    one-view launch (batch_views 1); requests that arrive while it runs wait
+    for it and then start an epoch of the persistent kernel together; lone
+    requests after an idle stop run plain again.  Every result equals the
+    plain launch of the same request bit for bit."""
+    hss, scs = three_scenes
+    big = (2, S.cam_orbit(hss[2].aabb, 1920, 1080, radius=25.0), 1920, 1080)
+    reqs = [big] + _requests(hss, 23, 25) + [big]
+    gaps = [0.0] * len(reqs)
+    gaps[0] = 0.02
+    gaps[1] = 0.001   # the scheduler takes the first request alone before the burst arrives
+    gaps[-1] = 0.05   # after the idle stop: a lone request again
+    print("[persistent] idle-plain launches and an epoch", flush=True)
+    results, st = _serve(dev, scs, reqs, gaps=gaps, epoch_idle_ms=2.0, idle_plain=True)
+    _check(dev, scs, reqs, results, "idle-plain")
+    views = [i.batch_views for _, i in results]
+    print(f"[persistent] requests per launch or epoch: {views}, launches + epochs {st['batches']}")
+    assert st["completed"] == len(reqs)
+    assert views[0] == 1 and views[-1] == 1, "a request on an idle device did not run alone"
+    assert max(views) > 1, "the burst did not start an epoch of the persistent kernel"
+    assert st["batches"] >= 3, st
