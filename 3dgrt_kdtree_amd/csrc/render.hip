@@ -570,20 +570,6 @@ __global__ __launch_bounds__(256) void k_valu_probe(float* __restrict__ out, int
     out[blockIdx.x * 256 + threadIdx.x] = r;
 }
 
-// Per-view cone records for the pre-test of rays leaving one origin e
-// (cone_pass).  For the sphere test's own f32 quantities a = c - e and
-// R = r + kappa*|a|_1 (same operations, same bits), sphere_pass accepting
-// implies, with eps = 2^-24 and exact arithmetic on the f32 inputs:
-//   |a x d| <= R' = R(1+4eps) + 10eps|a|   (rounding of the f32 cross product
-//                                           and of dist2 <= R*R),
-//   (a.d)^2 >= |a|^2 (1-16eps) - R'^2 =: Q^2  (|d|^2 >= 1-9eps for a
-//                                           normalised f32 direction),
-//   a.d > -R - 4eps|a|                      (t test with t_near >= 1e-2 > 0;
-//                                           f32 a.d is within 4eps|a|).
-// When Q >= R + 4eps|a| the second and third exclude a.d <= -Q, so
-// a.d >= Q and the f32 a.d >= q = Q - 4eps|a|.  q is formed in double and
-// rounded down; otherwise (origin near or inside the sphere, or r = inf) the
-// record accepts every direction.
 // First-launch estimate of the tile costs: each Gaussian centre in front of
 // the camera is projected to the image and counted in its density cell.  The
 // sort keys each tile by the centres in the 3 x 3 cells around it: where many
@@ -629,13 +615,111 @@ __global__ __launch_bounds__(kDensBlock) void k_density(const float4* __restrict
         if (hist[k]) atomicAdd(&dens[k], hist[k]);
 }
 
+// Cone records for the pre-test of rays leaving one origin e (cone_pass).
+// Bound-sphere bound: for the sphere test's own f32 quantities a = c - e and
+// R = r + kappa*|a|_1 (same operations, same bits), sphere_pass accepting
+// implies, with eps = 2^-24 and exact arithmetic on the f32 inputs:
+//   |a x d| <= R' = R(1+4eps) + 10eps|a|   (rounding of the f32 cross product
+//                                           and of dist2 <= R*R),
+//   (a.d)^2 >= |a|^2 (1-16eps) - R'^2 =: Q^2  (|d|^2 >= 1-9eps for a
+//                                           normalised f32 direction),
+//   a.d > -R - 4eps|a|                      (t test with t_near >= 1e-2 > 0;
+//                                           f32 a.d is within 4eps|a|).
+// When Q >= R + 4eps|a| the second and third exclude a.d <= -Q, so
+// a.d >= Q and the f32 a.d >= q = Q - 4eps|a|.  q is formed in double and
+// rounded down; otherwise (origin near or inside the sphere, or r = inf) the
+// record accepts every direction.
+//
+// Per-Gaussian cone record of a view from eye e: (a, q) with a = mu - e the
+// f32 difference and q a lower bound of the f32 dot product a.d (cone_pass,
+// three roundings) for every ray direction d from e whose exact test can
+// accept the Gaussian.  q is the larger of two bounds, both formed in double
+// and rounded down:
+//  * the bound-sphere bound above (r, kappa of the sphere record);
+//  * a bound from the projected ellipsoid.  A ray the exact test accepts
+//    passes within kappa |a|_1 (the f32 rounding of that test, as for the
+//    sphere) of the ellipsoid E = {x : (x - mu)^T S^-1 (x - mu) <= 1} scaled
+//    by 1.001 about mu, at a point in front of e, where S = 8 (A^T A)^-1 with
+//    A = diag(1/scale) M (sig holds S per Gaussian).  With v the unit vector
+//    from e to mu, that dilated ellipsoid lies in the cylinder of radius
+//    R = 1.001 sqrt(lambda_max(P S P)) + kappa |a|_1 about the line e + z v
+//    (P = I - v v^T: the largest semi-axis of E's projection across v) and in
+//    the slab |z - |mu - e|| <= h = 1.001 sqrt(v^T S v) + kappa |a|_1.  If
+//    z0 = |mu - e| - h > 0, an accepted direction makes an angle theta with v
+//    with tan theta <= R / z0, so (mu - e).d >= |mu - e| |d| C with
+//    C = z0 / sqrt(z0^2 + R^2); the f32 a.d then exceeds
+//    |mu - e| (C - 12 eps): eps for the rounding of a, 3 eps for the
+//    dot product and 4.5 eps for |d| >= 1 - 4.5 eps.  lambda_max(P S P) =
+//    (T + sqrt(2F - T^2)) / 2 with T = tr S - v^T S v and
+//    F = tr S^2 - 2 |S v|^2 + (v^T S v)^2 (the two nonzero eigenvalues of
+//    P S P have sum T and sum of squares F), plus 2e-7 tr S: sig holds S
+//    rounded to f32, and S <= S_f32 + 2^-24 |S|_F I <= S_f32 + 2^-24 tr(S) I
+//    (the doubled term also covers the double rounding here).
+// sig holds NaN where the sphere record has r = inf (a singular or
+// non-orthogonal rotation): the ellipsoid bound is skipped there.
+__global__ __launch_bounds__(256) void k_gaussian_cone(const float4* __restrict__ gsph, const float* __restrict__ sig,
+                                                       uint64_t n, float ex, float ey, float ez,
+                                                       float4* __restrict__ gcone) {
+    const double eps = 5.9604644775390625e-08;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+        const float4 sp = gsph[i];
+        const float ax = sp.x - ex, ay = sp.y - ey, az = sp.z - ez;
+        const uint32_t w = __float_as_uint(sp.w);
+        const float r = __half2float(__ushort_as_half((unsigned short)(w & 0xFFFFu)));
+        const float kap = __half2float(__ushort_as_half((unsigned short)(w >> 16)));
+        const float l1 = fabsf(ax) + fabsf(ay) + fabsf(az);
+        const float R = r + kap * l1;
+        double qd = -INFINITY;
+        if (R < 3.402823466e+38f) {
+            // bound sphere
+            const double a2 = (double)ax * ax + (double)ay * ay + (double)az * az;
+            const double na = sqrt(a2);
+            const double Rp = (double)R * (1.0 + 4.0 * eps) + 10.0 * eps * na;
+            const double Q2 = a2 * (1.0 - 16.0 * eps) - Rp * Rp;
+            const double lim = (double)R + 4.0 * eps * na;
+            if (Q2 > 0.0 && Q2 >= lim * lim) qd = sqrt(Q2) - 4.0 * eps * na;
+            // projected ellipsoid
+            const float* S = sig + 6 * i;
+            const double s00 = S[0], s01 = S[1], s02 = S[2], s11 = S[3], s12 = S[4], s22 = S[5];
+            const double mx = (double)sp.x - (double)ex, my = (double)sp.y - (double)ey, mz = (double)sp.z - (double)ez;
+            const double nm = sqrt(mx * mx + my * my + mz * mz);
+            if (s00 == s00 && nm > 0.0) {
+                const double inm = 1.0 / nm;
+                const double vx = mx * inm, vy = my * inm, vz = mz * inm;
+                const double t0 = s00 * vx + s01 * vy + s02 * vz, t1 = s01 * vx + s11 * vy + s12 * vz,
+                             t2 = s02 * vx + s12 * vy + s22 * vz;
+                const double vsv = fmax(vx * t0 + vy * t1 + vz * t2, 0.0);
+                const double tr = s00 + s11 + s22;
+                const double trs2 = s00 * s00 + s11 * s11 + s22 * s22 + 2.0 * (s01 * s01 + s02 * s02 + s12 * s12);
+                const double T = tr - vsv;
+                const double F = trs2 - 2.0 * (t0 * t0 + t1 * t1 + t2 * t2) + vsv * vsv;
+                const double lmax = 0.5 * (fmax(T, 0.0) + sqrt(fmax(2.0 * F - T * T, 0.0))) + 2e-7 * tr;
+                const double dil = (double)kap * (double)l1;
+                const double Rc = 1.001 * sqrt(lmax) + dil;
+                const double h = 1.001 * sqrt(vsv + 2e-7 * tr) + dil;
+                const double z0 = nm - h;
+                if (z0 > 0.0) {
+                    const double C = z0 / sqrt(z0 * z0 + Rc * Rc) - 12.0 * eps;
+                    if (C > 0.0) qd = fmax(qd, nm * C * (1.0 - 1e-12));
+                }
+            }
+        }
+        gcone[i] = make_float4(ax, ay, az, qd == -INFINITY ? -3.402823466e+38f : __double2float_rd(qd));
+    }
+}
+
+// Per-view cone records in leaf-slot order: the record of each slot's
+// Gaussian (Gaussians are numbered by first appearance in the leaf lists, so
+// consecutive slots read nearby records), zero records in the padding slots
+// (their pre-test bits are masked off).
 // SORT: the first workgroup (dispatched first, so it starts at once) sorts
 // the tile order of the coming view launch instead (tile_sort_block) while
-// the others stream the cone records, so the sort adds no time of its own.
+// the others copy the cone records, so the sort adds no time of its own.
 template <bool SORT>
-__global__ __launch_bounds__(SORT ? kCinBlock : kCinBlockSmall) void k_cone_prep(const float4* __restrict__ lsph,
-                                                                                 uint64_t n, float ex, float ey,
-                                                                                 float ez, float4* __restrict__ vcone,
+__global__ __launch_bounds__(SORT ? kCinBlock : kCinBlockSmall) void k_cone_prep(const uint32_t* __restrict__ offs,
+                                                                                 const float4* __restrict__ gcone,
+                                                                                 uint64_t n_slots, uint64_t n,
+                                                                                 float4* __restrict__ vcone,
                                                                                  TileSort ts) {
     const uint32_t first = SORT ? 1u : 0u, prep_blocks = gridDim.x - first;
     if (SORT && blockIdx.x < first) {
@@ -643,25 +727,8 @@ __global__ __launch_bounds__(SORT ? kCinBlock : kCinBlockSmall) void k_cone_prep
         return;
     }
     for (uint64_t i = (uint64_t)(blockIdx.x - first) * blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)prep_blocks * blockDim.x) {
-        const float4 sp = lsph[i];
-        const float ax = sp.x - ex, ay = sp.y - ey, az = sp.z - ez;
-        const uint32_t w = __float_as_uint(sp.w);
-        const float r = __half2float(__ushort_as_half((unsigned short)(w & 0xFFFFu)));
-        const float kap = __half2float(__ushort_as_half((unsigned short)(w >> 16)));
-        const float R = r + kap * (fabsf(ax) + fabsf(ay) + fabsf(az));
-        float q = -3.402823466e+38f;
-        if (R < 3.402823466e+38f) {
-            const double eps = 5.9604644775390625e-08;
-            const double a2 = (double)ax * ax + (double)ay * ay + (double)az * az;
-            const double na = sqrt(a2);
-            const double Rp = (double)R * (1.0 + 4.0 * eps) + 10.0 * eps * na;
-            const double Q2 = a2 * (1.0 - 16.0 * eps) - Rp * Rp;
-            const double lim = (double)R + 4.0 * eps * na;
-            if (Q2 > 0.0 && Q2 >= lim * lim) q = __double2float_rd(sqrt(Q2) - 4.0 * eps * na);
-        }
-        vcone[i] = make_float4(ax, ay, az, q);
-    }
+         i += (uint64_t)prep_blocks * blockDim.x)
+        vcone[i] = i < n_slots ? gcone[offs[i]] : make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // Batch queue counters, one per 128-B line: every wave of the chip claims
@@ -1114,6 +1181,9 @@ struct kdgrt_scene {
     // on one stream run in order; a launch on another stream first waits for
     // the last launch that used the table (cone_ev).
     void* d_vcone = nullptr;
+    void* d_gcone = nullptr; // the view's cone records per Gaussian (gathered into d_vcone)
+    void* d_gsig = nullptr;  // ellipsoid shape matrices per Gaussian, 6 floats each
+    void* d_ext = nullptr;   // the caller's Gaussian ID of each device number (debug outputs)
     hipEvent_t cone_ev = nullptr;
     hipStream_t cone_stream = nullptr;
     bool cone_used = false;
@@ -1257,7 +1327,11 @@ static void quat_to_mt(const float4& q, float m[3][3]) {
     m[2][2] = (1.f - 2.f * (xx + yy));
 }
 
-static float4 bound_sphere(const float4& a, const float4& b, const float4& q) {
+// Also writes S = 8 (A^T A)^-1, the shape matrix of the ellipsoid the exact
+// test accepts (x - mu)^T S^-1 (x - mu) <= 1, as (S00, S01, S02, S11, S12,
+// S22) to sig6, or NaN where r = inf.
+static float4 bound_sphere(const float4& a, const float4& b, const float4& q, double* sig6) {
+    for (int k = 0; k < 6; ++k) sig6[k] = NAN;
     float mf[3][3];
     quat_to_mt(q, mf);
     const double g[3] = {b.x, b.y, b.z};
@@ -1288,9 +1362,34 @@ static float4 bound_sphere(const float4& a, const float4& b, const float4& q) {
             const double smin = std::sqrt(lmin) * (1.0 - 1e-9), smax = std::sqrt(lmax);
             r = std::sqrt(8.0) / smin * (1.0 + 1e-3);
             kap = 8e-6 * (4.0 + smax / smin);
+            // 8 (A^T A)^-1 by the adjugate (A^T A is recomputed: the eigen
+            // solve overwrote it)
+            double P[3][3];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    double s2 = 0.0;
+                    for (int k = 0; k < 3; ++k) s2 += A[k][i] * A[k][j];
+                    P[i][j] = s2;
+                }
+            const double c00 = P[1][1] * P[2][2] - P[1][2] * P[2][1], c01 = P[0][2] * P[2][1] - P[0][1] * P[2][2],
+                         c02 = P[0][1] * P[1][2] - P[0][2] * P[1][1], c11 = P[0][0] * P[2][2] - P[0][2] * P[2][0],
+                         c12 = P[0][2] * P[1][0] - P[0][0] * P[1][2], c22 = P[0][0] * P[1][1] - P[0][1] * P[1][0];
+            const double det = P[0][0] * c00 + P[0][1] * (P[1][2] * P[2][0] - P[1][0] * P[2][2]) +
+                               P[0][2] * (P[1][0] * P[2][1] - P[1][1] * P[2][0]);
+            if (det > 0.0 && std::isfinite(det)) {
+                const double f = 8.0 / det;
+                const double S6[6] = {f * c00, f * c01, f * c02, f * c11, f * c12, f * c22};
+                bool fin = true;
+                for (int k = 0; k < 6; ++k) fin = fin && std::isfinite(S6[k]);
+                if (fin)
+                    for (int k = 0; k < 6; ++k) sig6[k] = S6[k];
+            }
         }
     }
-    if (!std::isfinite(r)) kap = 0.0;
+    if (!std::isfinite(r)) {
+        kap = 0.0;
+        for (int k = 0; k < 6; ++k) sig6[k] = NAN;
+    }
     const uint32_t packed = (uint32_t)f32_to_f16_up((float)std::min(r, 1e30)) |
                             ((uint32_t)f32_to_f16_up((float)std::min(kap, 1e30)) << 16);
     float w;
@@ -1497,7 +1596,8 @@ kdgrt_status kdgrt_framebuffer_to_u8(kdgrt_context* ctx, const float* fb, int wi
 uint64_t kdgrt_scene_bytes_estimate(uint32_t n_nodes, uint64_t n_offsets, uint64_t n_gaussians) {
     auto r = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
     return r(8ull * n_nodes) + r(48ull * n_nodes) + r(4ull * n_offsets) + r(80ull * std::max<uint64_t>(1, n_offsets)) +
-           r(192ull * n_gaussians) + 2 * r(16ull * (n_offsets + kLsphPad)) + r(16ull * n_gaussians);
+           r(192ull * n_gaussians) + 2 * r(16ull * (n_offsets + kLsphPad)) + r(16ull * n_gaussians) +
+           r(16ull * n_gaussians) + r(24ull * n_gaussians) + r(4ull * n_gaussians);
 }
 
 kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uint32_t n_nodes,
@@ -1509,15 +1609,40 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
     kdgrt_status st = kdgrt::validate_tree(nodes, n_nodes, n_offsets, offsets, n_g);
     if (st != KDGRT_OK) return st;
     HIP_TRY(hipSetDevice(ctx->device));
-    // Repack: 80-B geometry records (built per Gaussian, stored per leaf
-    // slot), 192-B SH records, 16-B bound spheres.
+    // Device numbering of the Gaussians: by first appearance in the leaf
+    // lists (Gaussians no leaf lists follow in their own order).  Leaves are
+    // stored depth first, so the records of Gaussians that neighbouring rays
+    // meet lie close together; the composite order is the leaf-slot order and
+    // does not depend on the numbering, and the debug outputs map back to the
+    // caller's IDs (ext).
+    std::vector<uint32_t> order(n_g), rank(n_g, UINT32_MAX), ioffs(n_offsets);
+    {
+        uint32_t next = 0;
+        for (uint64_t k = 0; k < n_offsets; ++k) {
+            const uint32_t id = offsets[k];
+            if (rank[id] == UINT32_MAX) {
+                rank[id] = next;
+                order[next++] = id;
+            }
+            ioffs[k] = rank[id];
+        }
+        for (uint64_t i = 0; i < n_g; ++i)
+            if (rank[i] == UINT32_MAX) {
+                rank[i] = next;
+                order[next++] = (uint32_t)i;
+            }
+    }
+    // Repack (device numbering): 80-B geometry records (built per Gaussian,
+    // stored per leaf slot), 192-B SH records, 16-B bound spheres.
     std::vector<float4> geom(5 * n_g), shade(12 * n_g), sph(n_g);
+    std::vector<double> sig(6 * n_g); // ellipsoid shape matrices of the view cone bound
+    std::vector<float> sigf(6 * n_g);
     // leaf-slot spheres plus kLsphPad zero records: the pre-test reads whole
     // groups of four past a leaf's end and masks the extra bits off
     std::vector<float4> lsph(n_offsets + kLsphPad, make_float4(0.f, 0.f, 0.f, 0.f));
     auto repack = [&](uint64_t i0, uint64_t i1) {
     for (uint64_t i = i0; i < i1; ++i) {
-        const kdgrt_gaussian& s = g[i];
+        const kdgrt_gaussian& s = g[order[i]];
         float inv[3], kinv;
         if (scales_inverted) {
             inv[0] = s.scale[0];
@@ -1551,12 +1676,12 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
             for (int k = 0; k < 15; ++k) sh[16 * ch + 1 + k] = s.f_rest[15 * ch + k];
         }
         sph[i] = bound_sphere(make_float4(s.pos[0], s.pos[1], s.pos[2], s.opacity),
-                              make_float4(inv[0], inv[1], inv[2], kinv), q);
+                              make_float4(inv[0], inv[1], inv[2], kinv), q, &sig[6 * i]);
     }
     };
     parallel_ranges(n_g, repack);
     parallel_ranges(n_offsets, [&](uint64_t k0, uint64_t k1) {
-        for (uint64_t k = k0; k < k1; ++k) lsph[k] = sph[offsets[k]]; // leaf-slot order
+        for (uint64_t k = k0; k < k1; ++k) lsph[k] = sph[ioffs[k]]; // leaf-slot order
     });
     auto* sc = new kdgrt_scene();
     sc->ctx = ctx;
@@ -1569,6 +1694,9 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
         if (sc->d_lsph) hipFree(sc->d_lsph);
         if (sc->d_gsph) hipFree(sc->d_gsph);
         if (sc->d_vcone) hipFree(sc->d_vcone);
+        if (sc->d_gsig) hipFree(sc->d_gsig);
+        if (sc->d_gcone) hipFree(sc->d_gcone);
+        if (sc->d_ext) hipFree(sc->d_ext);
         if (sc->cone_ev) hipEventDestroy(sc->cone_ev);
         delete sc;
         return set_error(e == hipErrorOutOfMemory ? KDGRT_ERR_OUT_OF_MEMORY : KDGRT_ERR_DEVICE,
@@ -1583,6 +1711,14 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
     if (n_g && (e = hipMemcpy(sc->d_gsph, sph.data(), 16ull * n_g, hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "copy");
     if ((e = hipMalloc(&sc->d_vcone, 16ull * lsph.size())) != hipSuccess) return fail(e, "view cones");
+    if ((e = hipMalloc(&sc->d_gcone, 16ull * std::max<uint64_t>(1, n_g))) != hipSuccess) return fail(e, "view cones");
+    for (uint64_t k = 0; k < 6 * n_g; ++k) sigf[k] = (float)sig[k];
+    if ((e = hipMalloc(&sc->d_gsig, 24ull * std::max<uint64_t>(1, n_g))) != hipSuccess) return fail(e, "shapes");
+    if ((e = hipMalloc(&sc->d_ext, 4ull * std::max<uint64_t>(1, n_g))) != hipSuccess) return fail(e, "IDs");
+    if (n_g && (e = hipMemcpy(sc->d_ext, order.data(), 4ull * n_g, hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(e, "copy");
+    if (n_g && (e = hipMemcpy(sc->d_gsig, sigf.data(), 24ull * n_g, hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(e, "copy");
     if ((e = hipEventCreateWithFlags(&sc->cone_ev, hipEventDisableTiming)) != hipSuccess) return fail(e, "event");
     if ((e = hipMemcpy(sc->d_lsph, lsph.data(), 16ull * lsph.size(), hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "copy");
@@ -1609,16 +1745,17 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
         if ((e = hipMemcpy(sc->d_quads, quads.data(), 16ull * quads.size(), hipMemcpyHostToDevice)) != hipSuccess)
             return fail(e, "copy");
     }
-    if (n_offsets && (e = hipMemcpy(sc->d_offs, offsets, 4ull * n_offsets, hipMemcpyHostToDevice)) != hipSuccess)
+    if (n_offsets && (e = hipMemcpy(sc->d_offs, ioffs.data(), 4ull * n_offsets, hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "copy");
     {
-        // geometry records in leaf-slot order, the Gaussian ID in the free last word
+        // geometry records in leaf-slot order, the device number of the
+        // Gaussian in the free last word
         std::vector<float4> sgeom(5 * std::max<uint64_t>(1, n_offsets));
         parallel_ranges(n_offsets, [&](uint64_t k0, uint64_t k1) {
             for (uint64_t k = k0; k < k1; ++k) {
-                for (int q = 0; q < 5; ++q) sgeom[5 * k + q] = geom[5 * (uint64_t)offsets[k] + q];
+                for (int q = 0; q < 5; ++q) sgeom[5 * k + q] = geom[5 * (uint64_t)ioffs[k] + q];
                 float w;
-                std::memcpy(&w, &offsets[k], 4);
+                std::memcpy(&w, &ioffs[k], 4);
                 sgeom[5 * k + 4].w = w;
             }
         });
@@ -1631,6 +1768,7 @@ kdgrt_status kdgrt_scene_create(kdgrt_context* ctx, const kdgrt_node* nodes, uin
     sc->dev.nodes = (const uint2*)sc->d_nodes;
     sc->dev.quads = (const uint4*)sc->d_quads;
     sc->dev.offs = (const uint32_t*)sc->d_offs;
+    sc->dev.ext = (const uint32_t*)sc->d_ext;
     sc->dev.sgeom = (const float4*)sc->d_sgeom;
     sc->dev.shade = (const float4*)sc->d_shade;
     sc->dev.lsph = (const float4*)sc->d_lsph;
@@ -1664,6 +1802,9 @@ kdgrt_status kdgrt_scene_destroy(kdgrt_scene* sc) {
     hipFree(sc->d_lsph);
     hipFree(sc->d_gsph);
     hipFree(sc->d_vcone);
+    hipFree(sc->d_gcone);
+    hipFree(sc->d_gsig);
+    hipFree(sc->d_ext);
     hipFree(sc->d_sched);
     hipEventDestroy(sc->cone_ev);
     delete sc;
@@ -1917,16 +2058,20 @@ static kdgrt_status with_cone_table(const kdgrt_scene* csc, ViewArgs& a, hipStre
         const kdgrt_status st = sched_before(sc, a, view_tiles(a), sched_kind, may_alloc, s, use, ts);
         if (st != KDGRT_OK) return st;
     }
+    const uint64_t ng = sc->n_gaussians;
+    const uint32_t g_grid = (uint32_t)std::min<uint64_t>((ng + 255) / 256, 2048u * 4u);
+    hipLaunchKernelGGL(k_gaussian_cone, dim3(g_grid), dim3(256), 0, s, (const float4*)sc->d_gsph,
+                       (const float*)sc->d_gsig, ng, a.cam.eye[0], a.cam.eye[1], a.cam.eye[2], (float4*)sc->d_gcone);
     const uint64_t n = sc->n_offsets + kLsphPad;
     const uint32_t blk = ts.on ? kCinBlock : kCinBlockSmall;
     const uint32_t prep_grid =
         (uint32_t)std::min<uint64_t>((n + blk - 1) / blk, (2048u * kCinBlock) / blk) + (ts.on ? 1u : 0u);
     if (ts.on)
-        hipLaunchKernelGGL(k_cone_prep<true>, dim3(prep_grid), dim3(blk), 0, s, (const float4*)sc->d_lsph, n,
-                           a.cam.eye[0], a.cam.eye[1], a.cam.eye[2], (float4*)sc->d_vcone, ts);
+        hipLaunchKernelGGL(k_cone_prep<true>, dim3(prep_grid), dim3(blk), 0, s, (const uint32_t*)sc->d_offs,
+                           (const float4*)sc->d_gcone, sc->n_offsets, n, (float4*)sc->d_vcone, ts);
     else
-        hipLaunchKernelGGL(k_cone_prep<false>, dim3(prep_grid), dim3(blk), 0, s, (const float4*)sc->d_lsph, n,
-                           a.cam.eye[0], a.cam.eye[1], a.cam.eye[2], (float4*)sc->d_vcone, ts);
+        hipLaunchKernelGGL(k_cone_prep<false>, dim3(prep_grid), dim3(blk), 0, s, (const uint32_t*)sc->d_offs,
+                           (const float4*)sc->d_gcone, sc->n_offsets, n, (float4*)sc->d_vcone, ts);
     a.sc.vcone = (const float4*)sc->d_vcone;
     const kdgrt_status st = launch(sc);
     if (st != KDGRT_OK) return st;

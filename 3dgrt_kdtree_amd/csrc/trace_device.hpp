@@ -52,7 +52,8 @@ namespace kdgrt_dev {
 struct DevScene {
     const uint2* __restrict__ nodes;
     const uint4* __restrict__ quads;  // per child pair c: nodes c, c+1, then the child pairs of both (48 B)
-    const uint32_t* __restrict__ offs;
+    const uint32_t* __restrict__ offs; // device number of the Gaussian in each leaf slot
+    const uint32_t* __restrict__ ext;  // the caller's ID of each device number (debug outputs only)
     const float4* __restrict__ shade; // 12 x float4 per Gaussian, channel-major (f_dc[c], f_rest[15c..15c+14])
     const float4* __restrict__ lsph;  // bound spheres in leaf-slot order: centre, packed (radius, kappa) as 2 x f16
     const float4* __restrict__ vcone; // per-view cone records in leaf-slot order (views only, else null)
@@ -210,10 +211,11 @@ __device__ __forceinline__ bool sphere_pass(const float o[3], const float d[3], 
 
 // Pre-test for rays that share the origin o of a view.  The record holds
 // a = centre - o (the f32 difference the sphere test forms) and a threshold
-// q: every candidate that sphere_pass accepts has a.d >= q, so testing
-// a.d >= q instead keeps all of them (q is derived and rounded down on the
-// device by k_cone_prep; its margin 4 eps |a| covers the rounding of this
-// dot product).
+// q: every direction the exact test can accept has a.d >= q, so testing
+// a.d >= q keeps all of them.  q (k_gaussian_cone, formed in double and
+// rounded down) is the tighter of the bound sphere's cone and the cone
+// around the Gaussian's ellipsoid projected across the view direction; its
+// margins cover the rounding of this dot product.
 __device__ __forceinline__ bool cone_pass(const float d[3], const float4 c) {
     const float tp = __builtin_fmaf(c.z, d[2], __builtin_fmaf(c.y, d[1], c.x * d[0]));
     return tp >= c.w;
@@ -646,7 +648,7 @@ __device__ __forceinline__ void leaf(RayState& st, const DevScene& sc, RayDbg* d
             }
             if (DBG) {
                 dbg->cnt[6]++;
-                if (dbg->t_eval) dbg->t_eval[kid[i]] = 1;
+                if (dbg->t_eval) dbg->t_eval[sc.ext[kid[i]]] = 1;
             }
             // View kernels (cone pre-test) request the 192-B SH record before
             // the response is formed, so its latency hides behind kd_expf
@@ -685,9 +687,10 @@ __device__ __forceinline__ void leaf(RayState& st, const DevScene& sc, RayDbg* d
             st.A = st.A + alpha * (1.0f - st.A);
             if (DBG) {
                 dbg->cnt[4]++;
-                if (dbg->t_blend) dbg->t_blend[gid] = 1;
-                dbg->hash = (dbg->hash ^ (uint64_t)gid) * 0x100000001b3ull;
-                if (dbg->seq && dbg->nseq < dbg->seq_cap) dbg->seq[dbg->nseq] = gid;
+                const uint32_t xid = sc.ext[gid]; // the caller's ID
+                if (dbg->t_blend) dbg->t_blend[xid] = 1;
+                dbg->hash = (dbg->hash ^ (uint64_t)xid) * 0x100000001b3ull;
+                if (dbg->seq && dbg->nseq < dbg->seq_cap) dbg->seq[dbg->nseq] = xid;
                 dbg->nseq++;
             }
             if (st.A > 0.99f) done = true;

@@ -556,11 +556,15 @@ def main():
     mean_c = cnt.reshape(-1, K.N_COUNTERS).mean(0)
     log("mean counters/ray: " + ", ".join(f"{n}={v:.2f}" for n, v in zip(K.COUNTER_NAMES, mean_c)))
     # Compulsory bytes of one render call: k_view's records read once each
-    # (touch maps of the debug kernel, same rays and tree) plus k_cone_prep's
-    # pass over the leaf slots (16-B bound sphere read, 16-B cone record written).
+    # (touch maps of the debug kernel, same rays and tree) plus the cone
+    # tables: k_gaussian_cone reads each Gaussian's 16-B bound sphere and
+    # 24-B shape matrix and writes its 16-B cone record; k_cone_prep reads
+    # each slot's 4-B device number and the Gaussian's record once and writes
+    # the slot's 16-B cone record.
     touch = dev.render_touch(sc, ci, W, H)
     comp = K.compulsory_bytes(touch, tree.offsets, rays)
-    cone_prep = 2 * 16 * (int(st["n_offsets"]) + 4)
+    n_slots, n_gauss = int(st["n_offsets"]) + 4, len(g)
+    cone_prep = (16 + 24 + 16) * n_gauss + (4 + 16) * n_slots + 16 * n_gauss
     comp_launch = comp["this_layout"] + cone_prep
     achieved = comp_launch / (ms_launch * 1e-3) / 1e9
     traffic = load_profile("traffic.json", "plain_1080p_1M")
@@ -573,7 +577,7 @@ def main():
             "compulsory_reference_layout_bytes": comp["reference_layout"],
             "logical_bytes_per_launch": B, "logical_bytes_per_ray": round(B / rays, 1),
             "logical_GB_s": round(B / (ms_launch * 1e-3) / 1e9, 1),
-            "kernel": "k_cone_prep + k_view<8, false, 0, false, true> (one render call; k_cone_prep also sorts the "
+            "kernel": "k_gaussian_cone + k_cone_prep + k_view<8, false, 0, false, true> (one render call; k_cone_prep also sorts the "
                       "tile order)"}
     log(f"roofline: {roof}")
 
