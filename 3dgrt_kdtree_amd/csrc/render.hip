@@ -330,6 +330,14 @@ struct RaysArgs {
 // uniform origins and directions: 313 -> 423 Mrays/s in this order).
 constexpr int kRayKeyBits = 30;
 constexpr uint64_t kRaySortMin = 65536;
+// Batched views of at least this many pixels (the 1920x1080 class) get cone
+// records of their own, up to kBatchConeMaxBytes of tables per batch:
+// preparing them takes about 70 us per view of a 1M-Gaussian scene, which
+// a 1280x720 view does not win back (64-view batch: 47.88 ms with sphere
+// pre-tests only, 47.33 ms with cone records for 1080p views, 47.52 ms for
+// 720p views as well).
+constexpr uint64_t kBatchConeMinPixels = 1500000;
+constexpr uint64_t kBatchConeMaxBytes = 16ull << 30;
 
 // Static scratch of kdgrt_render_rays_noalloc: two key buffers and two index
 // buffers of kNoAllocMaxRays entries, then the radix sort's temporaries
@@ -818,7 +826,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 4 
             }
             if (live) {
                 float rgb[3], A;
-                trace_ray<K, false>(it.sc, o, d, a.depth, wave_lds, lane, rgb, A, nullptr);
+                // a view with cone records of its own takes the view
+                // pre-test (the item, hence the branch, is wave-uniform)
+                if (it.sc.vcone)
+                    trace_ray<K, false, false, true, false, true, false>(it.sc, o, d, a.depth, wave_lds, lane, rgb, A,
+                                                                         nullptr);
+                else trace_ray<K, false>(it.sc, o, d, a.depth, wave_lds, lane, rgb, A, nullptr);
                 it.out[dst] = make_float4(rgb[0], rgb[1], rgb[2], 1.0f - A);
             }
         }
@@ -1233,6 +1246,16 @@ struct kdgrt_batch {
         size_t temp;
     };
     std::vector<SortJob> sorts;
+    // cone records of the batch's large views (the view pre-test), prepared
+    // by every run ahead of the persistent launch: one table per view, one
+    // per-Gaussian scratch table shared by the preparations in stream order
+    struct ConeJob {
+        const kdgrt_scene* scene;
+        float eye[3];
+        float4* table;
+    };
+    std::vector<ConeJob> cones;
+    void* d_gcone = nullptr;
 };
 
 using kdgrt::set_error;
@@ -2485,6 +2508,8 @@ kdgrt_status kdgrt_batch_destroy(kdgrt_batch* b) {
     hipFree(b->d_counters);
     for (void* p : b->d_perms) hipFree(p);
     for (auto& j : b->sorts) hipFree(j.ws);
+    for (auto& j : b->cones) hipFree(j.table);
+    hipFree(b->d_gcone);
     delete b;
     return KDGRT_OK;
 }
@@ -2586,6 +2611,25 @@ kdgrt_status batch_build(kdgrt_context* ctx, const kdgrt_view* views, int n_view
             w.perm = (const uint32_t*)perm;
         }
     }
+    if (e == hipSuccess) {
+        // large views use the view pre-test: cone records per view, within a
+        // memory cap (further large views keep the sphere pre-test)
+        uint64_t cone_bytes = 0, max_g = 0;
+        for (int i = 0; i < n_views && e == hipSuccess; ++i) {
+            const kdgrt_view& v = views[i];
+            const uint64_t bytes = 16ull * (v.scene->n_offsets + kLsphPad);
+            if ((uint64_t)v.width * v.height < kBatchConeMinPixels || cone_bytes + bytes > kBatchConeMaxBytes)
+                continue;
+            void* table = nullptr;
+            if ((e = hipMalloc(&table, bytes)) != hipSuccess) break;
+            cone_bytes += bytes;
+            max_g = std::max<uint64_t>(max_g, v.scene->n_gaussians);
+            kdgrt_batch::ConeJob j{v.scene, {v.cam.eye[0], v.cam.eye[1], v.cam.eye[2]}, (float4*)table};
+            b->cones.push_back(j);
+            items[i].sc.vcone = (const float4*)table;
+        }
+        if (e == hipSuccess && max_g) e = hipMalloc(&b->d_gcone, 16ull * max_g);
+    }
     if (e == hipSuccess) e = hipMalloc(&b->d_items, items.size() * sizeof(WorkItem));
     if (e == hipSuccess) e = hipMalloc(&b->d_counters, kCounterBytes);
     if (e == hipSuccess) e = hipMemcpy(b->d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice);
@@ -2595,6 +2639,8 @@ kdgrt_status batch_build(kdgrt_context* ctx, const kdgrt_view* views, int n_view
         hipFree(b->d_counters);
         for (void* p : b->d_perms) hipFree(p);
         for (auto& j : b->sorts) hipFree(j.ws);
+        for (auto& j : b->cones) hipFree(j.table);
+        hipFree(b->d_gcone);
         delete b;
         return set_error(KDGRT_ERR_DEVICE, std::string("batch_create: ") + hipGetErrorString(e));
     }
@@ -2619,6 +2665,19 @@ kdgrt_status batch_build(kdgrt_context* ctx, const kdgrt_view* views, int n_view
 }
 
 kdgrt_status batch_launch(kdgrt_batch* b, hipStream_t s) {
+    for (const auto& j : b->cones) {
+        const kdgrt_scene* sc = j.scene;
+        const uint64_t ng = sc->n_gaussians, n = sc->n_offsets + kLsphPad;
+        hipLaunchKernelGGL(k_gaussian_cone, dim3((uint32_t)std::min<uint64_t>((ng + 255) / 256, 8192)), dim3(256), 0,
+                           s, (const float4*)sc->d_gsph, (const float*)sc->d_gsig, ng, j.eye[0], j.eye[1], j.eye[2],
+                           (float4*)b->d_gcone);
+        hipLaunchKernelGGL(k_cone_prep<false>, dim3((uint32_t)std::min<uint64_t>((n + kCinBlockSmall - 1) /
+                                                                                  kCinBlockSmall,
+                                                                                  (2048u * kCinBlock) / kCinBlockSmall)),
+                           dim3(kCinBlockSmall), 0, s, (const uint32_t*)sc->d_offs, (const float4*)b->d_gcone,
+                           sc->n_offsets, n, j.table, TileSort{});
+        HIP_TRY(hipGetLastError());
+    }
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, b->d_counters, kXcds);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(b->kernel, dim3(b->grid), dim3(b->block), stack_lds_bytes(b->args.depth, b->block), s,

@@ -509,7 +509,8 @@ __device__ __forceinline__ void descend_once(RayState& st, const DevScene& sc, i
 // and early termination.  The K-entry register buffer holds the leaf's hits
 // by (t, slot); a leaf with more than K hits is finished by further passes
 // that keep only keys above the last composited one.
-template <int K, bool DBG, bool PROF, bool CONE, bool HYB>
+// EARLY_SH: request the SH record before the response is formed (below).
+template <int K, bool DBG, bool PROF, bool CONE, bool HYB, bool EARLY_SH = CONE && !HYB>
 __device__ __forceinline__ void leaf(RayState& st, const DevScene& sc, RayDbg* dbg) {
     const uint32_t base = st.node.y, count = st.node.x >> 3;
     if (DBG) {
@@ -650,13 +651,13 @@ __device__ __forceinline__ void leaf(RayState& st, const DevScene& sc, RayDbg* d
                 dbg->cnt[6]++;
                 if (dbg->t_eval) dbg->t_eval[sc.ext[kid[i]]] = 1;
             }
-            // View kernels (cone pre-test) request the 192-B SH record before
-            // the response is formed, so its latency hides behind kd_expf
-            // and the basis; only samples that the skip rule drops (alpha <
-            // 1/255; grayDist < 8 keeps the response above 0.0113) waste the
-            // load.  The other kernels hold more state and would spill, so
-            // they request it after the skip rule.
-            constexpr bool kEarlySH = CONE && !HYB;
+            // The view kernel (cone pre-test) requests the 192-B SH record
+            // before the response is formed, so its latency hides behind
+            // kd_expf and the basis; only samples that the skip rule drops
+            // (alpha < 1/255; grayDist < 8 keeps the response above 0.0113)
+            // waste the load.  The other kernels hold more state and would
+            // spill, so they request it after the skip rule (EARLY_SH false).
+            constexpr bool kEarlySH = EARLY_SH;
             const uint32_t gid = kid[i];
             const float4* shp = sc.shade + 12 * (size_t)gid;
             float4 shq[kEarlySH ? 12 : 1];
@@ -725,7 +726,8 @@ __host__ __device__ constexpr size_t wave_lds_bytes(int depth, bool dbg = false)
 // not reset them between segments), t_scene_far is clamped to max_t, and
 // cone_rt selects the cone pre-test at run time (the first segment of a view
 // ray) while later segments use the sphere pre-test.
-template <int K, bool DBG, bool PROF = false, bool CONE = false, bool HYB = false, bool AHEAD = true>
+template <int K, bool DBG, bool PROF = false, bool CONE = false, bool HYB = false, bool AHEAD = true,
+          bool EARLY_SH = CONE && !HYB>
 __device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], const float d[3], int depth,
                                           uint2* __restrict__ wave_lds, int lane, float rgb[3], float& A,
                                           RayDbg* dbg, float max_t = 0.0f, bool cone_rt = false) {
@@ -763,7 +765,7 @@ __device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], 
         while (true) {
             descend<DBG>(st, sc, depth, stk, dbg);
             KD_PROF_MARK(0);
-            leaf<K, DBG, PROF, CONE, HYB>(st, sc, dbg);
+            leaf<K, DBG, PROF, CONE, HYB, EARLY_SH>(st, sc, dbg);
             const bool more = next_leaf<DBG>(st, sc, depth, stk, dbg);
             KD_PROF_MARK(4);
             if (!more) break;
@@ -798,7 +800,7 @@ __device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], 
                 // the parked values are read back from LDS, not carried in
                 // registers through the composite
                 asm volatile("" ::: "memory");
-                leaf<K, DBG, PROF, CONE, HYB>(st, sc, dbg);
+                leaf<K, DBG, PROF, CONE, HYB, EARLY_SH>(st, sc, dbg);
                 asm volatile("" ::: "memory");
                 st.node = make_uint2(stash[0], stash[64]);
                 st.t_near = __uint_as_float(stash[128]);

@@ -132,8 +132,10 @@ def test_batch_matches_plain(dev):
     a = S.clustered(100000)
     b = S.uniform_10k()
     sa, sb = dev.scene(a.tree, a.g), dev.scene(b.tree, b.g)
+    # the two 1920x1080 views (one per scene) get cone records of their own,
+    # prepared by every run into per-view tables through one shared scratch
     specs = [(sa, a, (320, 240), (0.48, 0.36, 0.8)), (sb, b, (96, 64), None), (sa, a, (160, 120), (-0.2, 0.9, 0.1)),
-             (sb, b, (50, 37), None)]
+             (sb, b, (50, 37), None), (sa, a, (1920, 1080), (0.3, -0.5, 0.8)), (sb, b, (1920, 1080), None)]
     views, outs, refs = [], [], []
     for sc, hs, (w, h), direction in specs:
         ci = S.cam_orbit(hs.aabb, w, h, direction=direction) if direction else S.cam_c0(w, h)
