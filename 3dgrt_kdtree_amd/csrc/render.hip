@@ -323,12 +323,14 @@ struct RaysArgs {
     uint8_t* touch[6];    // DBG: touch maps as in ViewArgs
 };
 
-// Sort key of an explicit ray: its direction octant (3 bits), then the Morton
-// code of its origin in the scene box (9 bits per axis).  Rays sorted by it
-// share a wave with rays that start nearby and head the same way, so the
-// lanes of a wave walk more of the same nodes and leaves (2M rays with
-// uniform origins and directions: 313 -> 423 Mrays/s in this order).
-constexpr int kRayKeyBits = 30;
+// Sort key of an explicit ray: its direction cell (4 bits: the octahedral
+// map of the direction cut into 4 x 4 cells), then the Morton code of its
+// origin in the scene box (9 bits per axis).  Rays sorted by it share a wave
+// with rays that start nearby and head the same way, so the lanes of a wave
+// walk more of the same nodes and leaves (2M rays with uniform origins and
+// directions: 313 -> 423 Mrays/s with the direction octant in place of the
+// cell; 4M such rays on the 1M scene: 8.58 -> 8.19 ms with the 16 cells).
+constexpr int kRayKeyBits = 31;
 constexpr uint64_t kRaySortMin = 65536;
 // Batched views of at least this many pixels (the 1920x1080 class) get cone
 // records of their own, up to kBatchConeMaxBytes of tables per batch:
@@ -370,8 +372,21 @@ __global__ __launch_bounds__(256) void k_ray_keys(const float* __restrict__ ro, 
         };
         const uint32_t m = spread3(cell((ox - lo.x) * inv_ext.x)) | (spread3(cell((oy - lo.y) * inv_ext.y)) << 1) |
                            (spread3(cell((oz - lo.z) * inv_ext.z)) << 2);
-        const uint32_t oct = (dx < 0.0f ? 4u : 0u) | (dy < 0.0f ? 2u : 0u) | (dz < 0.0f ? 1u : 0u);
-        keys[i] = (oct << 27) | m;
+        // direction cell: the octahedral map of d (|d|_1 = 1 folded onto the
+        // square [-1, 1]^2) cut into 4 x 4 cells
+        const float sa = fabsf(dx) + fabsf(dy) + fabsf(dz);
+        float px = sa > 0.0f ? dx / sa : 0.0f, py = sa > 0.0f ? dy / sa : 0.0f;
+        if (dz < 0.0f) {
+            const float fx = (1.0f - fabsf(py)) * (px < 0.0f ? -1.0f : 1.0f);
+            const float fy = (1.0f - fabsf(px)) * (py < 0.0f ? -1.0f : 1.0f);
+            px = fx;
+            py = fy;
+        }
+        auto dcell = [](float v) { // [-1, 1] -> 0..3, clamped (NaN -> 0)
+            const float c = (v * 0.5f + 0.5f) * 4.0f;
+            return (uint32_t)(c > 0.0f ? (c < 3.0f ? c : 3.0f) : 0.0f);
+        };
+        keys[i] = (((dcell(py) << 2) | dcell(px)) << 27) | m;
         idx[i] = i;
     }
 }

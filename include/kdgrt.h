@@ -274,8 +274,9 @@ KDGRT_API kdgrt_status kdgrt_render_profile(kdgrt_context* ctx, const kdgrt_scen
                                             const kdgrt_camera_info* cam, int width, int height, float* out_rgbt,
                                             uint64_t* phase_cycles, void* stream);
 /* Explicit rays (device float3 arrays, directions used as given).  Lists of
- * 65,536 rays or more are traced in the order of a key (direction octant,
- * then the Morton code of the origin in the scene box), sorted on the device
+ * 65,536 rays or more are traced in the order of a key (direction cell of
+ * the octahedral map, then the Morton code of the origin in the scene box),
+ * sorted on the device
  * with scratch from the stream-ordered allocator; results are written at the
  * caller's indices and do not depend on the order. */
 KDGRT_API kdgrt_status kdgrt_render_rays(kdgrt_context* ctx, const kdgrt_scene* scene, const float* origins,
