@@ -2636,14 +2636,25 @@ kdgrt_status batch_build(kdgrt_context* ctx, const kdgrt_view* views, int n_view
             if ((uint64_t)v.width * v.height < kBatchConeMinPixels || cone_bytes + bytes > kBatchConeMaxBytes)
                 continue;
             void* table = nullptr;
-            if ((e = hipMalloc(&table, bytes)) != hipSuccess) break;
+            if (hipMalloc(&table, bytes) != hipSuccess) {
+                // the tables are optional: with too little free memory the
+                // remaining views keep the sphere pre-test
+                (void)hipGetLastError();
+                break;
+            }
             cone_bytes += bytes;
             max_g = std::max<uint64_t>(max_g, v.scene->n_gaussians);
             kdgrt_batch::ConeJob j{v.scene, {v.cam.eye[0], v.cam.eye[1], v.cam.eye[2]}, (float4*)table};
             b->cones.push_back(j);
             items[i].sc.vcone = (const float4*)table;
         }
-        if (e == hipSuccess && max_g) e = hipMalloc(&b->d_gcone, 16ull * max_g);
+        if (max_g && hipMalloc(&b->d_gcone, 16ull * max_g) != hipSuccess) {
+            (void)hipGetLastError();
+            b->d_gcone = nullptr;
+            for (auto& j : b->cones) hipFree(j.table);
+            b->cones.clear();
+            for (WorkItem& w : items) w.sc.vcone = nullptr;
+        }
     }
     if (e == hipSuccess) e = hipMalloc(&b->d_items, items.size() * sizeof(WorkItem));
     if (e == hipSuccess) e = hipMalloc(&b->d_counters, kCounterBytes);
