@@ -748,7 +748,7 @@ __device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], 
             go = !(st.t_near > tf) && (st.A < 0.99f);
         }
     }
-    // Look-ahead (AHEAD; never in the counting, profiling and hybrid variants): the
+    // Look-ahead (AHEAD; never in the counting and profiling variants): the
     // lanes of a wave descend in lock step and wait for the lane with the
     // longest descent before any of them composites its leaf.  A lane that
     // has found its leaf takes it (its node and t range), makes the step
@@ -760,7 +760,7 @@ __device__ __forceinline__ void trace_ray(const DevScene& sc, const float o[3], 
     // iterations in which its lane would have idled.  Each lane visits the
     // same leaves in the same order, so results are unchanged.  While it
     // composites, a lane parks the state of its next descent in its LDS stash.
-    constexpr bool kAhead = AHEAD && !DBG && !PROF && !HYB;
+    constexpr bool kAhead = AHEAD && !DBG && !PROF;
     if (go && !kAhead) {
         while (true) {
             descend<DBG>(st, sc, depth, stk, dbg);
