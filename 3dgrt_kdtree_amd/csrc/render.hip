@@ -176,6 +176,9 @@ constexpr int kSchedClasses = 512;
 constexpr uint32_t kCinBlock = 1024;
 constexpr uint32_t kCinBlockSmall = 256;
 constexpr uint32_t kSchedMaxTiles = 1u << 20;
+constexpr int kPrepUnroll = 8;
+constexpr uint32_t kConeCuBlocks = 16;
+constexpr uint32_t kPrepCuThreads = 4096;
 
 __device__ __forceinline__ uint32_t sched_class(uint32_t c) {
     if (c == 0) return 0;
@@ -246,9 +249,12 @@ __device__ void tile_sort_block(const TileSort& ts) {
         }
     }
     __syncthreads();
-    // eight elements per thread and step, loaded before any is used, so a
-    // step waits for one memory round trip rather than eight
-    constexpr uint32_t kU = 8;
+    // sixteen elements per thread and step, loaded before any is used, so a
+    // step waits for one memory round trip rather than sixteen, and a 1080p
+    // view (32,640 logical tiles) takes two steps per pass: the sort runs
+    // beside the cone-record copy, whose traffic makes every round trip long
+    // (eight per thread and step: the sort outlasted the copy by about 13 us)
+    constexpr uint32_t kU = 16;
     const uint32_t stride = kU * blockDim.x;
     for (uint32_t b0 = 0; !ts.density && b0 < n; b0 += stride) {
         uint32_t c[kU], e[kU];
@@ -680,55 +686,87 @@ __global__ __launch_bounds__(kDensBlock) void k_density(const float4* __restrict
 //    (the doubled term also covers the double rounding here).
 // sig holds NaN where the sphere record has r = inf (a singular or
 // non-orthogonal rotation): the ellipsoid bound is skipped there.
+__device__ __forceinline__ float4 gaussian_cone(const float4 sp, const float (&S)[6], float ex, float ey, float ez) {
+    const double eps = 5.9604644775390625e-08;
+    const float ax = sp.x - ex, ay = sp.y - ey, az = sp.z - ez;
+    const uint32_t w = __float_as_uint(sp.w);
+    const float r = __half2float(__ushort_as_half((unsigned short)(w & 0xFFFFu)));
+    const float kap = __half2float(__ushort_as_half((unsigned short)(w >> 16)));
+    const float l1 = fabsf(ax) + fabsf(ay) + fabsf(az);
+    const float R = r + kap * l1;
+    double qd = -INFINITY;
+    if (R < 3.402823466e+38f) {
+        // bound sphere
+        const double a2 = (double)ax * ax + (double)ay * ay + (double)az * az;
+        const double na = sqrt(a2);
+        const double Rp = (double)R * (1.0 + 4.0 * eps) + 10.0 * eps * na;
+        const double Q2 = a2 * (1.0 - 16.0 * eps) - Rp * Rp;
+        const double lim = (double)R + 4.0 * eps * na;
+        if (Q2 > 0.0 && Q2 >= lim * lim) qd = sqrt(Q2) - 4.0 * eps * na;
+        // projected ellipsoid
+        const double s00 = S[0], s01 = S[1], s02 = S[2], s11 = S[3], s12 = S[4], s22 = S[5];
+        const double mx = (double)sp.x - (double)ex, my = (double)sp.y - (double)ey, mz = (double)sp.z - (double)ez;
+        const double nm = sqrt(mx * mx + my * my + mz * mz);
+        if (s00 == s00 && nm > 0.0) {
+            const double inm = 1.0 / nm;
+            const double vx = mx * inm, vy = my * inm, vz = mz * inm;
+            const double t0 = s00 * vx + s01 * vy + s02 * vz, t1 = s01 * vx + s11 * vy + s12 * vz,
+                         t2 = s02 * vx + s12 * vy + s22 * vz;
+            const double vsv = fmax(vx * t0 + vy * t1 + vz * t2, 0.0);
+            const double tr = s00 + s11 + s22;
+            const double trs2 = s00 * s00 + s11 * s11 + s22 * s22 + 2.0 * (s01 * s01 + s02 * s02 + s12 * s12);
+            const double T = tr - vsv;
+            const double F = trs2 - 2.0 * (t0 * t0 + t1 * t1 + t2 * t2) + vsv * vsv;
+            const double lmax = 0.5 * (fmax(T, 0.0) + sqrt(fmax(2.0 * F - T * T, 0.0))) + 2e-7 * tr;
+            const double dil = (double)kap * (double)l1;
+            const double Rc = 1.001 * sqrt(lmax) + dil;
+            const double h = 1.001 * sqrt(vsv + 2e-7 * tr) + dil;
+            const double z0 = nm - h;
+            if (z0 > 0.0) {
+                const double C = z0 / sqrt(z0 * z0 + Rc * Rc) - 12.0 * eps;
+                if (C > 0.0) qd = fmax(qd, nm * C * (1.0 - 1e-12));
+            }
+        }
+    }
+    return make_float4(ax, ay, az, qd == -INFINITY ? -3.402823466e+38f : __double2float_rd(qd));
+}
+
+// Grid-stride over the Gaussians (about one per thread for a 1M-Gaussian
+// scene, gaussian_cone_grid), the records of a thread's next Gaussian
+// requested before the current one's double-precision arithmetic.
 __global__ __launch_bounds__(256) void k_gaussian_cone(const float4* __restrict__ gsph, const float* __restrict__ sig,
                                                        uint64_t n, float ex, float ey, float ez,
                                                        float4* __restrict__ gcone) {
-    const double eps = 5.9604644775390625e-08;
-    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
-        const float4 sp = gsph[i];
-        const float ax = sp.x - ex, ay = sp.y - ey, az = sp.z - ez;
-        const uint32_t w = __float_as_uint(sp.w);
-        const float r = __half2float(__ushort_as_half((unsigned short)(w & 0xFFFFu)));
-        const float kap = __half2float(__ushort_as_half((unsigned short)(w >> 16)));
-        const float l1 = fabsf(ax) + fabsf(ay) + fabsf(az);
-        const float R = r + kap * l1;
-        double qd = -INFINITY;
-        if (R < 3.402823466e+38f) {
-            // bound sphere
-            const double a2 = (double)ax * ax + (double)ay * ay + (double)az * az;
-            const double na = sqrt(a2);
-            const double Rp = (double)R * (1.0 + 4.0 * eps) + 10.0 * eps * na;
-            const double Q2 = a2 * (1.0 - 16.0 * eps) - Rp * Rp;
-            const double lim = (double)R + 4.0 * eps * na;
-            if (Q2 > 0.0 && Q2 >= lim * lim) qd = sqrt(Q2) - 4.0 * eps * na;
-            // projected ellipsoid
-            const float* S = sig + 6 * i;
-            const double s00 = S[0], s01 = S[1], s02 = S[2], s11 = S[3], s12 = S[4], s22 = S[5];
-            const double mx = (double)sp.x - (double)ex, my = (double)sp.y - (double)ey, mz = (double)sp.z - (double)ez;
-            const double nm = sqrt(mx * mx + my * my + mz * mz);
-            if (s00 == s00 && nm > 0.0) {
-                const double inm = 1.0 / nm;
-                const double vx = mx * inm, vy = my * inm, vz = mz * inm;
-                const double t0 = s00 * vx + s01 * vy + s02 * vz, t1 = s01 * vx + s11 * vy + s12 * vz,
-                             t2 = s02 * vx + s12 * vy + s22 * vz;
-                const double vsv = fmax(vx * t0 + vy * t1 + vz * t2, 0.0);
-                const double tr = s00 + s11 + s22;
-                const double trs2 = s00 * s00 + s11 * s11 + s22 * s22 + 2.0 * (s01 * s01 + s02 * s02 + s12 * s12);
-                const double T = tr - vsv;
-                const double F = trs2 - 2.0 * (t0 * t0 + t1 * t1 + t2 * t2) + vsv * vsv;
-                const double lmax = 0.5 * (fmax(T, 0.0) + sqrt(fmax(2.0 * F - T * T, 0.0))) + 2e-7 * tr;
-                const double dil = (double)kap * (double)l1;
-                const double Rc = 1.001 * sqrt(lmax) + dil;
-                const double h = 1.001 * sqrt(vsv + 2e-7 * tr) + dil;
-                const double z0 = nm - h;
-                if (z0 > 0.0) {
-                    const double C = z0 / sqrt(z0 * z0 + Rc * Rc) - 12.0 * eps;
-                    if (C > 0.0) qd = fmax(qd, nm * C * (1.0 - 1e-12));
-                }
-            }
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float4 sp = gsph[i];
+    float S[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) S[k] = sig[6 * i + k];
+    for (;;) {
+        const uint64_t j = i + stride;
+        float4 sp_n = make_float4(0.f, 0.f, 0.f, 0.f);
+        float S_n[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (j < n) {
+            sp_n = gsph[j];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) S_n[k] = sig[6 * j + k];
         }
-        gcone[i] = make_float4(ax, ay, az, qd == -INFINITY ? -3.402823466e+38f : __double2float_rd(qd));
+        gcone[i] = gaussian_cone(sp, S, ex, ey, ez);
+        if (j >= n) break;
+        i = j;
+        sp = sp_n;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) S[k] = S_n[k];
     }
+}
+
+// Workgroups of a k_gaussian_cone pass over n Gaussians: kConeCuBlocks
+// workgroups of 256 threads per CU, fewer for a small scene.
+static uint32_t gaussian_cone_grid(uint64_t n, int n_cu) {
+    const uint64_t want = (n + 255) / 256, cap = (uint64_t)std::max(1, n_cu) * kConeCuBlocks;
+    return (uint32_t)std::max<uint64_t>(1, std::min(want, cap));
 }
 
 // Per-view cone records in leaf-slot order: the record of each slot's
@@ -749,9 +787,36 @@ __global__ __launch_bounds__(SORT ? kCinBlock : kCinBlockSmall) void k_cone_prep
         tile_sort_block(ts);
         return;
     }
-    for (uint64_t i = (uint64_t)(blockIdx.x - first) * blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)prep_blocks * blockDim.x)
+    // Chunks of 64 x kPrepUnroll consecutive slots are dealt to the
+    // wavefronts in turn (whatever the workgroup size): a wave issues the ID
+    // loads of its chunk together, then its gathers, then its stores, so it
+    // keeps several dependent gathers in flight; the rest (and the padding
+    // slots, which get zero records) one record per lane at a time.
+    const uint32_t lane = threadIdx.x & 63u, wpb = blockDim.x / 64u;
+    const uint64_t wid = (uint64_t)(blockIdx.x - first) * wpb + threadIdx.x / 64u, nw = (uint64_t)prep_blocks * wpb;
+    constexpr uint64_t chunk = 64u * kPrepUnroll;
+    const uint64_t nfull = n_slots / chunk;
+    for (uint64_t c = wid; c < nfull; c += nw) {
+        const uint64_t b = c * chunk + lane;
+        uint32_t g[kPrepUnroll];
+#pragma unroll
+        for (int u = 0; u < kPrepUnroll; ++u) g[u] = __builtin_nontemporal_load(offs + b + 64u * u);
+        float4 v[kPrepUnroll];
+#pragma unroll
+        for (int u = 0; u < kPrepUnroll; ++u) v[u] = gcone[g[u]];
+#pragma unroll
+        for (int u = 0; u < kPrepUnroll; ++u) vcone[b + 64u * u] = v[u];
+    }
+    for (uint64_t i = nfull * chunk + wid * 64u + lane; i < n; i += nw * 64u)
         vcone[i] = i < n_slots ? gcone[offs[i]] : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Workgroups of a cone-table pass over n slot records: kPrepCuThreads
+// threads per CU, fewer when the table is short.
+static uint32_t cone_prep_grid(uint64_t n, uint32_t blk, int n_cu) {
+    const uint64_t want = (n + (uint64_t)blk * kPrepUnroll - 1) / ((uint64_t)blk * kPrepUnroll);
+    const uint64_t cap = (uint64_t)std::max(1, n_cu) * kPrepCuThreads / blk;
+    return (uint32_t)std::max<uint64_t>(1, std::min(want, cap));
 }
 
 // Batch queue counters, one per 128-B line: every wave of the chip claims
@@ -2097,13 +2162,12 @@ static kdgrt_status with_cone_table(const kdgrt_scene* csc, ViewArgs& a, hipStre
         if (st != KDGRT_OK) return st;
     }
     const uint64_t ng = sc->n_gaussians;
-    const uint32_t g_grid = (uint32_t)std::min<uint64_t>((ng + 255) / 256, 2048u * 4u);
+    const uint32_t g_grid = gaussian_cone_grid(ng, sc->ctx->prop.multiProcessorCount);
     hipLaunchKernelGGL(k_gaussian_cone, dim3(g_grid), dim3(256), 0, s, (const float4*)sc->d_gsph,
                        (const float*)sc->d_gsig, ng, a.cam.eye[0], a.cam.eye[1], a.cam.eye[2], (float4*)sc->d_gcone);
     const uint64_t n = sc->n_offsets + kLsphPad;
     const uint32_t blk = ts.on ? kCinBlock : kCinBlockSmall;
-    const uint32_t prep_grid =
-        (uint32_t)std::min<uint64_t>((n + blk - 1) / blk, (2048u * kCinBlock) / blk) + (ts.on ? 1u : 0u);
+    const uint32_t prep_grid = cone_prep_grid(n, blk, sc->ctx->prop.multiProcessorCount) + (ts.on ? 1u : 0u);
     if (ts.on)
         hipLaunchKernelGGL(k_cone_prep<true>, dim3(prep_grid), dim3(blk), 0, s, (const uint32_t*)sc->d_offs,
                            (const float4*)sc->d_gcone, sc->n_offsets, n, (float4*)sc->d_vcone, ts);
@@ -2694,12 +2758,11 @@ kdgrt_status batch_launch(kdgrt_batch* b, hipStream_t s) {
     for (const auto& j : b->cones) {
         const kdgrt_scene* sc = j.scene;
         const uint64_t ng = sc->n_gaussians, n = sc->n_offsets + kLsphPad;
-        hipLaunchKernelGGL(k_gaussian_cone, dim3((uint32_t)std::min<uint64_t>((ng + 255) / 256, 8192)), dim3(256), 0,
-                           s, (const float4*)sc->d_gsph, (const float*)sc->d_gsig, ng, j.eye[0], j.eye[1], j.eye[2],
+        hipLaunchKernelGGL(k_gaussian_cone, dim3(gaussian_cone_grid(ng, sc->ctx->prop.multiProcessorCount)), dim3(256),
+                           0, s, (const float4*)sc->d_gsph, (const float*)sc->d_gsig, ng, j.eye[0], j.eye[1], j.eye[2],
                            (float4*)b->d_gcone);
-        hipLaunchKernelGGL(k_cone_prep<false>, dim3((uint32_t)std::min<uint64_t>((n + kCinBlockSmall - 1) /
-                                                                                  kCinBlockSmall,
-                                                                                  (2048u * kCinBlock) / kCinBlockSmall)),
+        hipLaunchKernelGGL(k_cone_prep<false>,
+                           dim3(cone_prep_grid(n, kCinBlockSmall, sc->ctx->prop.multiProcessorCount)),
                            dim3(kCinBlockSmall), 0, s, (const uint32_t*)sc->d_offs, (const float4*)b->d_gcone,
                            sc->n_offsets, n, j.table, TileSort{});
         HIP_TRY(hipGetLastError());
