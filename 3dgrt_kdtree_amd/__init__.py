@@ -565,6 +565,21 @@ class Device:
         return {"kernel": "k_valu_probe", "waves": waves.value, "fma_per_wave": per.value, "ms": ms,
                 "valu_instr_per_s": waves.value * per.value / (ms * 1e-3)}
 
+    def view_cone_table(self, scene, eye):
+        """The cone records a plain view from `eye` pre-tests
+        (kdgrt_view_cone_table): float32 (n_offsets + 4, 4), rows (a, q) in
+        leaf-slot order, then four zero padding rows."""
+        e = (C.c_float * 3)(*[float(v) for v in eye])
+        n = C.c_uint64()
+        check(lib().kdgrt_view_cone_table(self.h, scene.h, e, None, C.c_uint64(0), C.byref(n), None))
+        out = self.alloc(n.value * 16)
+        try:
+            check(lib().kdgrt_view_cone_table(self.h, scene.h, e, out.ptr, C.c_uint64(n.value), C.byref(n), None))
+            self.sync()
+            return out.download(np.float32, (n.value, 4))
+        finally:
+            out.free()
+
     def render_touch(self, scene, ci, w, h, depth=12):
         """Touch maps of one plain view (kdgrt_render_touch): which node,
         quad, leaf-slot and Gaussian records the launch reads."""

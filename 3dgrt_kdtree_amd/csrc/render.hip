@@ -29,6 +29,7 @@
 #include <thread>
 #include <vector>
 
+#include "f16_up.hpp"
 #include "internal.hpp"
 #include "runtime.hpp"
 #include "secondary_device.hpp"
@@ -1405,13 +1406,6 @@ static void sym3_eigen(double a[3][3], double ev[3]) {
     for (int k = 0; k < 3; ++k) ev[k] = a[k][k];
 }
 
-static uint16_t f32_to_f16_up(float x) {
-    // smallest half >= x for x >= 0 (inf and NaN map to +inf)
-    if (!(x <= 65504.0f)) return 0x7C00u;
-    if (x <= 0.0f) return 0;
-    __half h = __float2half_ru(x);
-    return __half_as_ushort(h);
-}
 
 // quaternionWXYZToMatrixTranspose (cudaRenderer.cu:808-828), f32, q = (w, x, y, z).
 static void quat_to_mt(const float4& q, float m[3][3]) {
@@ -1493,8 +1487,14 @@ static float4 bound_sphere(const float4& a, const float4& b, const float4& q, do
         kap = 0.0;
         for (int k = 0; k < 6; ++k) sig6[k] = NAN;
     }
-    const uint32_t packed = (uint32_t)f32_to_f16_up((float)std::min(r, 1e30)) |
-                            ((uint32_t)f32_to_f16_up((float)std::min(kap, 1e30)) << 16);
+    // (float) of a double may round down: the f16 rounding starts from the
+    // next float up in that case
+    const auto up_f32 = [](double v) {
+        const float f = (float)v;
+        return (double)f < v ? std::nextafter(f, INFINITY) : f;
+    };
+    const uint32_t packed = (uint32_t)kdgrt::f32_to_f16_up(up_f32(std::min(r, 1e30))) |
+                            ((uint32_t)kdgrt::f32_to_f16_up(up_f32(std::min(kap, 1e30))) << 16);
     float w;
     std::memcpy(&w, &packed, 4);
     return make_float4(a.x, a.y, a.z, w);
@@ -2290,6 +2290,38 @@ kdgrt_status kdgrt_render_profile(kdgrt_context* ctx, const kdgrt_scene* sc, con
     fill_view_args(a, sc, cam, W, H, 12, out);
     a.prof = phase_cycles; // 8 x u64 per ray
     return launch_view(sc, a, k_view<kViewK, false, KDGRT_OUT_RGBT_F32, true>, (hipStream_t)stream);
+}
+
+// The two launches with_cone_table makes before a plain view, writing the
+// table into the caller's buffer instead of the scene's.
+kdgrt_status kdgrt_view_cone_table(kdgrt_context* ctx, const kdgrt_scene* csc, const float eye[3], void* out,
+                                   uint64_t capacity, uint64_t* n_records, void* stream) {
+    if (!ctx || !csc || !eye) return set_error(KDGRT_ERR_INVALID_ARGUMENT, "view_cone_table: null argument");
+    if (csc->ctx != ctx)
+        return set_error(KDGRT_ERR_INVALID_ARGUMENT, "view_cone_table: scene belongs to another context");
+    if (!std::isfinite(eye[0]) || !std::isfinite(eye[1]) || !std::isfinite(eye[2]))
+        return set_error(KDGRT_ERR_INVALID_ARGUMENT, "view_cone_table: eye not finite");
+    const uint64_t n = csc->n_offsets + kLsphPad;
+    if (n_records) *n_records = n;
+    if (!out) return KDGRT_OK;
+    if (capacity < n) return set_error(KDGRT_ERR_CAPACITY, "view_cone_table: capacity below n_offsets + 4 records");
+    kdgrt_scene* sc = const_cast<kdgrt_scene*>(csc);
+    const hipStream_t s = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lock(sc->cone_mu);
+    if (sc->cone_used && sc->cone_stream != s) HIP_TRY(hipStreamWaitEvent(s, sc->cone_ev, 0));
+    const int n_cu = sc->ctx->prop.multiProcessorCount;
+    if (sc->n_gaussians)
+        hipLaunchKernelGGL(k_gaussian_cone, dim3(gaussian_cone_grid(sc->n_gaussians, n_cu)), dim3(256), 0, s,
+                           (const float4*)sc->d_gsph, (const float*)sc->d_gsig, sc->n_gaussians, eye[0], eye[1],
+                           eye[2], (float4*)sc->d_gcone);
+    hipLaunchKernelGGL(k_cone_prep<false>, dim3(cone_prep_grid(n, kCinBlockSmall, n_cu)), dim3(kCinBlockSmall), 0, s,
+                       (const uint32_t*)sc->d_offs, (const float4*)sc->d_gcone, sc->n_offsets, n, (float4*)out,
+                       TileSort{});
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(sc->cone_ev, s));
+    sc->cone_stream = s;
+    sc->cone_used = true;
+    return KDGRT_OK;
 }
 
 kdgrt_status kdgrt_secondary_demo_spheres(kdgrt_secondary_sphere out[2]) {

@@ -273,6 +273,17 @@ KDGRT_API kdgrt_status kdgrt_render_rays_touch(kdgrt_context* ctx, const kdgrt_s
 KDGRT_API kdgrt_status kdgrt_render_profile(kdgrt_context* ctx, const kdgrt_scene* scene,
                                             const kdgrt_camera_info* cam, int width, int height, float* out_rgbt,
                                             uint64_t* phase_cycles, void* stream);
+/* Measurement helper, no reference counterpart (the pre-test data of a
+ * view): the cone records that a plain view from `eye` pre-tests, one per
+ * leaf slot in the order of the tree's offsets, then 4 zero records (the
+ * padding a pre-test group may read past the last leaf).  A record is four
+ * floats (a.x, a.y, a.z, q): a = mu - eye in f32 for the slot's Gaussian and
+ * q a lower bound of the f32 dot product a.d over every ray direction d from
+ * `eye` that the exact test can accept (-FLT_MAX: no bound).  *n_records
+ * receives n_offsets + 4; out (caller-owned device memory of `capacity`
+ * records) may be NULL to query it.  Asynchronous on `stream`. */
+KDGRT_API kdgrt_status kdgrt_view_cone_table(kdgrt_context* ctx, const kdgrt_scene* scene, const float eye[3],
+                                             void* out, uint64_t capacity, uint64_t* n_records, void* stream);
 /* Explicit rays (device float3 arrays, directions used as given).  Lists of
  * 65,536 rays or more are traced in the order of a key (direction cell of
  * the octahedral map, then the Morton code of the origin in the scene box),

@@ -35,6 +35,24 @@ def clustered(n, seed=1001):
     return HostScene(g, K.Tree.build(g))
 
 
+@functools.lru_cache(maxsize=None)
+def anisotropic(n=3000, seed=41):
+    """Needles and pancakes: one axis of each Gaussian 1, 10 or 100 times
+    the others, random rotations, in the unit cube."""
+    rng = np.random.default_rng(seed)
+    g = np.zeros(n, dtype=K.GAUSSIAN_DTYPE)
+    g["pos"] = rng.uniform(-1, 1, (n, 3))
+    s = np.exp(rng.normal(np.log(0.01), 0.6, (n, 3)))
+    s[np.arange(n), rng.integers(0, 3, n)] *= rng.choice([1.0, 10.0, 100.0], n)
+    g["scale"] = np.minimum(s, 0.5)
+    q = rng.normal(size=(n, 4))
+    g["rot"] = q / np.linalg.norm(q, axis=1, keepdims=True)
+    g["opacity"] = 1.0 / (1.0 + np.exp(-rng.normal(0, 1.5, n)))
+    g["k_scale"] = 1.0
+    g["f_dc"] = (rng.random((n, 3)) - 0.5) / 0.28209479177387814
+    return HostScene(g, K.Tree.build(g))
+
+
 def cam_c0(w=256, h=256):
     """Eye (0,0,3), n = (0,0,1), u = (1,0,0), v = (0,1,0), fovy 60, near 0.005."""
     c = K.Camera()

@@ -62,30 +62,13 @@ def test_camera_inside_the_box(dev):
     sc.free()
 
 
-def _anisotropic_scene(n=3000, seed=41):
-    """Needles and pancakes: one axis of each Gaussian 1, 10 or 100 times
-    the others, random rotations, in the unit cube."""
-    rng = np.random.default_rng(seed)
-    g = np.zeros(n, dtype=K.GAUSSIAN_DTYPE)
-    g["pos"] = rng.uniform(-1, 1, (n, 3))
-    s = np.exp(rng.normal(np.log(0.01), 0.6, (n, 3)))
-    s[np.arange(n), rng.integers(0, 3, n)] *= rng.choice([1.0, 10.0, 100.0], n)
-    g["scale"] = np.minimum(s, 0.5)
-    q = rng.normal(size=(n, 4))
-    g["rot"] = q / np.linalg.norm(q, axis=1, keepdims=True)
-    g["opacity"] = 1.0 / (1.0 + np.exp(-rng.normal(0, 1.5, n)))
-    g["k_scale"] = 1.0
-    g["f_dc"] = (rng.random((n, 3)) - 0.5) / 0.28209479177387814
-    return S.HostScene(g, K.Tree.build(g))
-
-
 @pytest.mark.parametrize("eye", [(0.0, 0.0, 1.6), (0.9, 0.7, 1.4), (4.0, 3.0, 6.0), (0.2, -0.1, 0.3)])
 def test_anisotropic_gaussians_view_cone(dev, eye):
     """The view pre-test's cone thresholds bound each Gaussian by its
     ellipsoid projected across the view direction: needle and pancake
     Gaussians seen from near (inside the cloud), middle and far eyes render
     exactly as the oracle's exact tests do."""
-    hs = _anisotropic_scene()
+    hs = S.anisotropic()
     cam = K.look_at(np.asarray(eye), np.zeros(3), 70.0, 4 / 3, 0.005)
     ci = K.camera_make(cam, 200, 150)
     sc = dev.scene(hs.tree, hs.g)
