@@ -191,6 +191,7 @@ __device__ __forceinline__ uint32_t sched_class(uint32_t c) {
 // Density cells of the first-launch estimate: cells of 2^cell_shift x
 // 2^cell_shift tiles, at most kDensCells of them (32 KB of LDS in the sort).
 constexpr uint32_t kDensCells = 8192;
+static_assert(kDensCells % kCinBlock == 0, "the sort loads its cells in whole steps");
 
 struct TileSort {
     const uint32_t* cost; // durations of the previous launch, per logical tile
@@ -208,16 +209,18 @@ struct TileSort {
 
 // Density key of logical tile t: Gaussian centres counted in the 3 x 3
 // cells around the tile's cell (0 outside the image).
-__device__ __forceinline__ uint32_t density_key(const TileSort& ts, const uint32_t* dl, uint32_t t) {
-    const uint32_t st = t / 64u, local = t % 64u;
-    const int tx = (int)(st % (uint32_t)ts.blocks_x) * 8 + (int)(local % 8u);
-    const int ty = (int)(st / (uint32_t)ts.blocks_x) * 8 + (int)(local / 8u);
+__device__ __forceinline__ uint32_t density_key(const TileSort& ts, const uint32_t* box, uint32_t t, float inv_bx) {
+    // box holds, per cell, the centre count of the 3x3 cells around it
+    const uint32_t st = t / 64u, local = t % 64u, bx = (uint32_t)ts.blocks_x;
+    // st / bx from the float reciprocal, exact after one correction either
+    // way (st < 2^24: the product is off by far less than one)
+    uint32_t gy = (uint32_t)((float)st * inv_bx);
+    if (gy * bx > st) --gy;
+    else if ((gy + 1u) * bx <= st) ++gy;
+    const int tx = (int)(st - gy * bx) * 8 + (int)(local % 8u);
+    const int ty = (int)gy * 8 + (int)(local / 8u);
     if (tx >= ts.tiles_w || ty >= ts.tiles_h) return 0;
-    const int cx = tx >> ts.cell_shift, cy = ty >> ts.cell_shift;
-    uint32_t k = 0;
-    for (int y = max(cy - 1, 0); y <= min(cy + 1, ts.cells_h - 1); ++y)
-        for (int x = max(cx - 1, 0); x <= min(cx + 1, ts.cells_w - 1); ++x) k += dl[y * ts.cells_w + x];
-    return k;
+    return box[(ty >> ts.cell_shift) * ts.cells_w + (tx >> ts.cell_shift)];
 }
 
 // The sort, run by one workgroup of kCinBlock threads.  Keys: the smoothed
@@ -232,12 +235,50 @@ __device__ void tile_sort_block(const TileSort& ts) {
     const uint32_t n = ts.n;
     const int reset = ts.reset;
     const uint32_t tid = threadIdx.x;
+    const float inv_bx = 1.0f / (float)max(ts.blocks_x, 1);
     for (uint32_t k = tid; k < (uint32_t)kSchedClasses; k += blockDim.x) cls[k] = 0;
     if (ts.density) {
-        const uint32_t nc = (uint32_t)(ts.cells_w * ts.cells_h);
-        for (uint32_t k = tid; k < nc; k += blockDim.x) {
-            dl[k] = ts.dens[k];
-            ts.dens[k] = 0; // ready for the next estimate
+        // the cell counts into LDS (at most kDensCells = kUc x kCinBlock:
+        // each thread loads its kUc cells at once, one round trip behind the
+        // cone-record copy's traffic) and zeroed for the next estimate; then,
+        // in place, each cell's count becomes the sum over the 3x3 cells
+        // around it (clamped at the borders): a horizontal, then a vertical
+        // pass, since the clamped window is a product of two intervals
+        const uint32_t nc = (uint32_t)(ts.cells_w * ts.cells_h), cw = (uint32_t)ts.cells_w;
+        constexpr uint32_t kUc = kDensCells / kCinBlock;
+        uint32_t v[kUc];
+#pragma unroll
+        for (uint32_t j = 0; j < kUc; ++j) {
+            const uint32_t k = j * blockDim.x + tid;
+            v[j] = k < nc ? ts.dens[k] : 0u;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kUc; ++j) {
+            const uint32_t k = j * blockDim.x + tid;
+            if (k < nc) {
+                dl[k] = v[j];
+                ts.dens[k] = 0;
+            }
+        }
+        for (uint32_t pass = 0; pass < 2; ++pass) {
+            __syncthreads();
+            const uint32_t step = pass == 0 ? 1u : cw;
+#pragma unroll
+            for (uint32_t j = 0; j < kUc; ++j) {
+                const uint32_t k = j * blockDim.x + tid;
+                if (k < nc) {
+                    const uint32_t x = k % cw, y = k / cw;
+                    const bool lo = pass == 0 ? x > 0u : y > 0u;
+                    const bool hi = pass == 0 ? x + 1u < cw : y + 1u < (uint32_t)ts.cells_h;
+                    v[j] = dl[k] + (lo ? dl[k - step] : 0u) + (hi ? dl[k + step] : 0u);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (uint32_t j = 0; j < kUc; ++j) {
+                const uint32_t k = j * blockDim.x + tid;
+                if (k < nc) dl[k] = v[j];
+            }
         }
         __syncthreads();
         constexpr uint32_t kUd = 8;
@@ -245,7 +286,7 @@ __device__ void tile_sort_block(const TileSort& ts) {
 #pragma unroll
             for (uint32_t j = 0; j < kUd; ++j) {
                 const uint32_t t = b0 + j * blockDim.x + tid;
-                if (t < n) atomicAdd(&cls[sched_class(density_key(ts, dl, t))], 1u);
+                if (t < n) atomicAdd(&cls[sched_class(density_key(ts, dl, t, inv_bx))], 1u);
             }
         }
     }
@@ -303,7 +344,7 @@ __device__ void tile_sort_block(const TileSort& ts) {
 #pragma unroll
         for (uint32_t j = 0; j < kU; ++j) {
             const uint32_t t = b0 + j * blockDim.x + tid;
-            v[j] = t < n ? (ts.density ? density_key(ts, dl, t) : smooth[t]) : 0u;
+            v[j] = t < n ? (ts.density ? density_key(ts, dl, t, inv_bx) : smooth[t]) : 0u;
         }
 #pragma unroll
         for (uint32_t j = 0; j < kU; ++j) {
