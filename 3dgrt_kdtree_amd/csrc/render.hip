@@ -84,6 +84,13 @@ static inline uint32_t interleave_grid(uint64_t n_logical) {
     return (uint32_t)(((n_logical + G * kXcds - 1) / (G * kXcds)) * (G * kXcds));
 }
 
+// Ray packets (64 rays) per XCD group of k_rays.  Runs of a sorted ray list
+// differ in cost (a run is one direction cell over part of the box), and
+// long runs per XCD leave some XCDs the costly ones: 4M incoherent rays on
+// the 1M scene take 8.00 ms with groups of 8, 8.14-8.19 with 64 and 9.5-9.7
+// with 256 or 1024 (1, 4 and 16: 8.04-8.11).
+constexpr uint32_t kRaysGroup = 8;
+
 template <int K, bool DBG, int FMT, bool PROF = false, bool CONE = true>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DBG || PROF ? 1 : 4))) void k_view(ViewArgs a) {
     // One wavefront per block = one 8x8-pixel tile, so a finished wave frees
@@ -441,7 +448,7 @@ __global__ __launch_bounds__(256) void k_ray_keys(const float* __restrict__ ro, 
 
 template <int K, bool DBG>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DBG ? 1 : 4))) void k_rays(RaysArgs a) {
-    const uint64_t j = (uint64_t)xcd_interleave<64>(blockIdx.x) * 64 + threadIdx.x;
+    const uint64_t j = (uint64_t)xcd_interleave<kRaysGroup>(blockIdx.x) * 64 + threadIdx.x;
     if (j >= a.n) return;
     const uint64_t i = a.perm ? (uint64_t)a.perm[j] : j;
     const float o[3] = {a.ro[3 * i], a.ro[3 * i + 1], a.ro[3 * i + 2]};
@@ -2475,7 +2482,7 @@ static kdgrt_status launch_rays(kdgrt_context* ctx, const kdgrt_scene* sc, const
     if (n > (uint64_t)0x7FFFF000 * 64ull) return set_error(KDGRT_ERR_CAPACITY, "render_rays: too many rays");
     RaysArgs a{sc->dev, ro, rd, n, depth, (float4*)out, counters, hashes, nullptr, {}};
     if (maps) touch_ptrs(maps, a.touch);
-    const dim3 grid(interleave_grid<64>((n + 63) / 64)), block(64);
+    const dim3 grid(interleave_grid<kRaysGroup>((n + 63) / 64)), block(64);
     hipStream_t s = (hipStream_t)stream;
     const size_t lds = stack_lds_bytes(depth, 64, dbg);
     // Lists of at least kRaySortMin rays are traced in ray-key order (the
@@ -2538,7 +2545,7 @@ kdgrt_status kdgrt_render_rays_noalloc(kdgrt_context* ctx, const kdgrt_scene* sc
         HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, temp, keys, idx, (uint32_t)n, 0, kRayKeyBits, s));
         a.perm = idx.Current();
     }
-    hipLaunchKernelGGL((k_rays<kViewK, false>), dim3(interleave_grid<64>((n + 63) / 64)), dim3(64),
+    hipLaunchKernelGGL((k_rays<kViewK, false>), dim3(interleave_grid<kRaysGroup>((n + 63) / 64)), dim3(64),
                        stack_lds_bytes(depth, 64), s, a);
     HIP_TRY(hipGetLastError());
     return KDGRT_OK;
