@@ -9,6 +9,9 @@ scene resident in HBM, plus
                 the measured HBM traffic beside it,
   valu          the VALU-issue rate of k_view against the measured ceiling of
                 a pure fused-multiply-add kernel,
+  texture       k_view's texture-path busy share and the share that its load
+                instructions fill at the path's 16-cycle floor per instruction
+                (counter runs summarised under profiles/),
   cpu_baseline  the FP32 oracle (CPU restatement of the reference
                 traversal+compositing, OpenMP) on this host's cores, and the
                 10k-Gaussian 256x256 case the reference runs on the CPU,
@@ -617,6 +620,17 @@ def main():
                      "kernel_valu_instr_per_launch": vprof["sq_insts_valu_per_dispatch"],
                      "source": vprof.get("source")})
     log(f"valu: {valu}")
+    # Texture-path ceiling, from the same counter runs: a 16-B vector load
+    # instruction holds the path for at least 16 cycles per CU whatever the
+    # number of active lanes (probe kernel), so reads x 16 / kernel cycles is
+    # the share of the kernel that the load instructions alone fill.
+    texture = None
+    if vprof is not None and "td_floor_frac" in vprof:
+        texture = {"bound": "texture path", "busy": round(vprof["td_busy"], 4),
+                   "cycles_per_read": round(vprof["td_cycles_per_read"], 2), "floor_cycles_per_read": 16,
+                   "reads_per_launch": vprof["vmem_reads_per_dispatch"],
+                   "floor_frac": round(vprof["td_floor_frac"], 4), "source": vprof.get("source")}
+    log(f"texture path: {texture}")
     result = {"metric": METRIC, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
               "warmup": args.warmup, "ms_per_step": round(ms_launch, 4), "higher_is_better": True, "scaling": "weak",
               "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -626,7 +640,7 @@ def main():
                                      f"previous launch's tile durations",
                          "gaussians": len(g), "width": W, "height": H, "stack_depth": 12,
                          "tile_order": "frame-to-frame schedule (raster-order rate under tile_order)"},
-              "device": info["name"], "roofline": roof, "valu": valu, "hbm_copy": copy,
+              "device": info["name"], "roofline": roof, "valu": valu, "texture": texture, "hbm_copy": copy,
               **({"phases": phases} if phases else {}),
               "build": {"host_sah_build_s": round(build_s, 2), "threads": os.cpu_count(), "nodes": st["n_nodes"],
                         "leaf_refs": st["n_offsets"], "proxy_triangles": st["n_triangles"]}}
