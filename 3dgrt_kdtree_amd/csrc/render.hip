@@ -1454,7 +1454,6 @@ static void sym3_eigen(double a[3][3], double ev[3]) {
     for (int k = 0; k < 3; ++k) ev[k] = a[k][k];
 }
 
-
 // quaternionWXYZToMatrixTranspose (cudaRenderer.cu:808-828), f32, q = (w, x, y, z).
 static void quat_to_mt(const float4& q, float m[3][3]) {
     const float qr = q.x, qx = q.y, qy = q.z, qz = q.w;

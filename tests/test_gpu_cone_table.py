@@ -19,7 +19,6 @@ everything only slows the view down.  So the table itself is checked:
 * a Gaussian without a bound (the eye inside its slab) has q = -FLT_MAX.
 """
 import importlib
-import math
 
 import numpy as np
 import pytest
