@@ -469,7 +469,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DBG ? 1 : 4)
         dbg.t_blend = a.touch[5];
     }
     extern __shared__ uint2 lds_stack[];
-    trace_ray<K, DBG>(a.sc, o, d, a.depth, lds_stack, (int)threadIdx.x, rgb, A, &dbg);
+    // SH records requested early (4M incoherent rays 7.59 -> 7.45 ms); the
+    // debug kernel keeps the late request
+    trace_ray<K, DBG, false, false, false, true, !DBG>(a.sc, o, d, a.depth, lds_stack, (int)threadIdx.x, rgb, A, &dbg);
     a.out[i] = make_float4(rgb[0], rgb[1], rgb[2], 1.0f - A);
     if (DBG) {
         if (a.counters)
@@ -956,14 +958,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 4 
             if (live) {
                 float rgb[3], A;
                 // a view with cone records of its own takes the view
-                // pre-test and requests SH records early, as the view kernel
-                // does (the item, hence the branch, is wave-uniform; 16 B of
-                // per-tile state spill, and the 64-view batch still gains
-                // 45.12 -> 44.77 ms)
+                // pre-test (the item, hence the branch, is wave-uniform); both
+                // branches request SH records early (16 B of per-tile state
+                // spill, and the 64-view batch still gains 45.12 -> 44.33 ms)
                 if (it.sc.vcone)
                     trace_ray<K, false, false, true, false, true, true>(it.sc, o, d, a.depth, wave_lds, lane, rgb, A,
                                                                          nullptr);
-                else trace_ray<K, false>(it.sc, o, d, a.depth, wave_lds, lane, rgb, A, nullptr);
+                else
+                    trace_ray<K, false, false, false, false, true, true>(it.sc, o, d, a.depth, wave_lds, lane, rgb, A,
+                                                                          nullptr);
                 it.out[dst] = make_float4(rgb[0], rgb[1], rgb[2], 1.0f - A);
             }
         }
