@@ -523,8 +523,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     for (int b = 0; b < ha.sec.max_bounces; ++b) {
         float t_hit;
         const int hit = sec_closest(ha.sec, o, d, t_hit);
-        trace_ray<K, false, false, false, true>(a.sc, o, d, a.depth, lds_stack, lane, acc, A, nullptr,
-                                                hit >= 0 ? t_hit : kSecFar, b == 0);
+        trace_ray<K, false, false, false, true, true, true>(a.sc, o, d, a.depth, lds_stack, lane, acc, A, nullptr,
+                                                            hit >= 0 ? t_hit : kSecFar, b == 0);
         for (int k = 0; k < 3; ++k) fin[k] = fin[k] + thr[k] * acc[k];
         for (int k = 0; k < 3; ++k) thr[k] = thr[k] * (1.0f - A);
         if (thr[0] < 0.01f && thr[1] < 0.01f && thr[2] < 0.01f) break;
@@ -1157,9 +1157,10 @@ __device__ __forceinline__ void serve_tile(const ServeArgs& a, const ServeRec& r
         if (x < r.W && y < r.H) {
             float o[3], d[3], rgb[3], A;
             make_ray(r.cam, x, y, o, d);
-            // without the look-ahead: with it this kernel's extra state spills
-            // (36 B per lane) and the 64-view burst takes 50.3 instead of 49.0 ms
-            trace_ray<K, false, false, false, false, false>(sc, o, d, a.depth, lds_stack, lane, rgb, A, nullptr);
+            // without the look-ahead (with it the 64-view burst takes 54.3
+            // instead of 48.6 ms: this kernel holds more state), SH records
+            // requested early (47.9 instead of 48.7 ms)
+            trace_ray<K, false, false, false, false, false, true>(sc, o, d, a.depth, lds_stack, lane, rgb, A, nullptr);
             uint32_t* px = reinterpret_cast<uint32_t*>(a.arena + r.out_off) + 4 * ((size_t)(r.H - y - 1) * r.W + x);
             st_system(px + 0, __float_as_uint(rgb[0]));
             st_system(px + 1, __float_as_uint(rgb[1]));

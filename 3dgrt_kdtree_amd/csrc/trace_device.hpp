@@ -651,13 +651,12 @@ __device__ __forceinline__ void leaf(RayState& st, const DevScene& sc, RayDbg* d
                 dbg->cnt[6]++;
                 if (dbg->t_eval) dbg->t_eval[sc.ext[kid[i]]] = 1;
             }
-            // The view, batch and ray-list kernels request the 192-B SH
-            // record before the response is formed, so its latency hides
-            // behind kd_expf and the basis; only samples that the skip rule
-            // drops (alpha < 1/255; grayDist < 8 keeps the response above
-            // 0.0113) waste the load.  The serving and hybrid kernels hold
-            // more state and would spill more, so they request it after the
-            // skip rule (EARLY_SH false).
+            // With EARLY_SH (every kernel but the ray-list debug kernel) the
+            // 192-B SH record is requested before the response is formed, so
+            // its latency hides behind kd_expf and the basis; only samples
+            // that the skip rule drops (alpha < 1/255; grayDist < 8 keeps the
+            // response above 0.0113) waste the load.  Without it the record
+            // is read after the skip rule.
             constexpr bool kEarlySH = EARLY_SH;
             const uint32_t gid = kid[i];
             const float4* shp = sc.shade + 12 * (size_t)gid;
