@@ -42,10 +42,13 @@ namespace {
 constexpr int kXcds = 8;
 constexpr int kMaxDevices = 64;
 
-// Hit-buffer capacity of the plain view and ray-list launches (0.013 % of
-// the leaf visits of the 1M-Gaussian 1080p view have more hits and take a
-// second pass).
-constexpr int kViewK = 8;
+// Hit-buffer capacity of the view, ray-list, hybrid and serving kernels: a
+// leaf visit with more hits takes a second, exact pass.  7 rather than 8
+// (1080p view -1.1 %, 4M rays -1.6 %, hybrid view -3.3 %, measured with the
+// iterative instruction scheduler; 6 gains less, 10 loses 6 %): one entry
+// fewer frees registers the scheduler uses better than it loses to the rare
+// extra passes.  The batch kernel keeps its own choice of 4, 8 or 16.
+constexpr int kViewK = 7;
 
 // ---------------------------------------------------------------------------
 // Plain one-view kernel.
@@ -2604,7 +2607,7 @@ kdgrt_status kdgrt_tuning_default(kdgrt_context* ctx, kdgrt_tuning* t) {
     t->tile_w = 8;
     t->tile_h = 8;
     t->stack_depth = 12;
-    t->hit_capacity = kViewK;
+    t->hit_capacity = 8; // the batch kernel is built for 4, 8 and 16
     t->waves_per_cu = 32;
     t->block_threads = 64;
     return KDGRT_OK;
