@@ -64,7 +64,7 @@ extern "C" kdgrt_status kdgrt_autotune(kdgrt_context* ctx, const kdgrt_view* vie
     const std::vector<Dim> dims = {
         {0, {8, 16, 24, 32, 48}}, // waves per CU
         {1, {0, 1, 2}},           // tile: 8x8, 16x8, 16x16
-        {2, {4, 8, 16}},          // hit capacity
+        {2, {4, 7, 8, 16}},       // hit capacity
         {3, {12, 8, 16}},         // stack depth
         {4, {64, 128, 256}},      // block threads
     };

@@ -395,7 +395,7 @@ typedef struct kdgrt_tuning {
     int32_t tile_w;          /* ray tile width in pixels (tile = one work-queue item) */
     int32_t tile_h;
     int32_t stack_depth;     /* 1..32 */
-    int32_t hit_capacity;    /* per-ray sorted hit buffer: 4, 8 or 16 */
+    int32_t hit_capacity;    /* per-ray sorted hit buffer: 4, 7, 8 or 16 */
     int32_t waves_per_cu;    /* persistent grid = waves_per_cu * CUs * 64 lanes / block size */
     int32_t block_threads;   /* 64, 128 or 256 */
 } kdgrt_tuning;

@@ -185,7 +185,7 @@ def test_autotune_small(dev):
     out = dev.alloc(128 * 128 * 16)
     best, ms = K.autotune(dev, [K.View(sc.h, ci, 128, 128, out.ptr)], max_trials=6, reps=2)
     assert ms > 0
-    assert best.hit_capacity in (4, 8, 16) and 1 <= best.stack_depth <= 32
+    assert best.hit_capacity in (4, 7, 8, 16) and 1 <= best.stack_depth <= 32
     sc.free()
 
 

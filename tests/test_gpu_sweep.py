@@ -15,7 +15,7 @@ composited-ID hash and sequence, RGB and transmittance), the plain view
 kernel must write the same image, and the ray list (plain and debug) must
 equal the oracle's ray-list render.  The view and the ray list then run
 together as one persistent batch under a random tuning (tile shape, hit
-buffer 4, 8 or 16 with its overflow re-pass, workgroup size, occupancy,
+buffer 4, 7, 8 or 16 with its overflow re-pass, workgroup size, occupancy,
 with or without graph replay), whose results must equal the oracle's too.
 """
 import importlib
@@ -125,7 +125,7 @@ def test_random_scene_view_and_rays(dev, stream, seed):
     # the same view and rays as one persistent batch, random tuning
     tw, th = [(8, 8), (16, 8), (16, 16)][int(rng.integers(0, 3))]
     threads = int(rng.choice([64, 128])) if depth > 16 else int(rng.choice([64, 128, 256]))
-    tun = K.Tuning(tw, th, depth, int(rng.choice([4, 8, 16])), int(rng.choice([8, 16, 32, 48])), threads)
+    tun = K.Tuning(tw, th, depth, int(rng.choice([4, 7, 8, 16])), int(rng.choice([8, 16, 32, 48])), threads)
     outv, outr = dev.alloc(w * h * 16), dev.alloc(len(o) * 16)
     ob, db = dev.upload(o), dev.upload(d)
     b = K.Batch(dev, [K.View(sc.h, ci, w, h, outv.ptr)], [K.RaySet(sc.h, ob.ptr, db.ptr, len(o), outr.ptr)], tun)
