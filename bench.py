@@ -580,7 +580,7 @@ def main():
             "compulsory_reference_layout_bytes": comp["reference_layout"],
             "logical_bytes_per_launch": B, "logical_bytes_per_ray": round(B / rays, 1),
             "logical_GB_s": round(B / (ms_launch * 1e-3) / 1e9, 1),
-            "kernel": "k_gaussian_cone + k_cone_prep + k_view<8, false, 0, false, true> (one render call; k_cone_prep also sorts the "
+            "kernel": "k_gaussian_cone + k_cone_prep + k_view<7, false, 0, false, true> (one render call; k_cone_prep also sorts the "
                       "tile order)"}
     log(f"roofline: {roof}")
 
