@@ -1,6 +1,6 @@
 // ORACLE — test infrastructure only.  Nothing in the product links or calls
 // this file; only tests/, __graft_entry__.smoke() and bench.py's CPU
-// baseline leg load liboracle.so, as the checker.
+// baseline leg load liboracle.so, to compare results against.
 //
 // A CPU restatement of the reference's plain render path
 // (renderKernelGaussian_sortNode, SECONDARY_RAY=0, PRIMITIVE_TYPE

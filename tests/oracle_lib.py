@@ -1,4 +1,4 @@
-"""ctypes loader for the CPU oracle (oracle/liboracle.so) — the checker.
+"""ctypes loader for the CPU oracle (oracle/liboracle.so), which device results are compared against.
 
 Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use it.
 """

@@ -1,5 +1,5 @@
 """Literal, slow restatement of the reference's ELLIPSOID_BY_TRI kd-tree build
-(test infrastructure: the checker for the fast builder on small scenes).
+(test infrastructure: the fast builder is compared against it on small scenes).
 
 Follows Kd-treeConstructor.cpp statement by statement (paths relative to
 Kd-treeConverter_v0.9/Kd-treeConverter/):
